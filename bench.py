@@ -1,0 +1,235 @@
+#!/usr/bin/env python3
+"""Throughput of the iterative-reconstruction hot path on one MI355X.
+
+Workload (default): MLEM on a stack of 64 slices of 2048 x 2048 fp32 seeded
+ellipse phantoms (20 ellipses each, seed 2), 1500 angles evenly spaced in
+[0, pi), 2048 detector columns, Poisson noise at 1e4 incident counts --
+the 2048^2 / 1500-angle geometry the metric is quoted on.  Inputs are
+resident in HBM before timing starts.
+
+A step is one full iteration over the whole stack (zero accumulators, the
+ray pass A x -> weights -> A^T w, image update).  `value` is slice-iterations
+per second (= iterations per second per 2048^2 slice, summed over slices).
+
+`roofline` reports the dominant kernel (the ray pass): algorithmic bytes per
+launch = slices x (16 nnz + 12 R + 8 N^2) where nnz is the exact segment
+count of one slice (counted by the traversal) and R = P x C rays, divided by
+the kernel's average launch time measured with HIP events on the launch
+stream; peak 8 TB/s.
+
+`cpu_baseline` times the reference's own threaded CPU engine (compiled from
+its sources into oracle/_ref) on this host: one MLEM iteration of one
+2048^2 slice over an evenly spaced subset of the angles, scaled linearly to
+the full angle count.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("SIRT/MLEM iter/s per 2048² slice; batched slices/s vs plain at "
+          "≤1e-4 rel err")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(N, P_full, threads, target_s=12.0):
+    """Reference CPU engine (oracle/_ref) on a bounded sample; returns a
+    dict or None when the compiled reference is absent."""
+    try:
+        from oracle import load_ref, ref_available
+    except Exception:
+        return None
+    if not ref_available():
+        return None
+    import numpy as np
+    from trace_amd.phantom import angles_deg
+    R = load_ref()
+    rng = np.random.default_rng(0)
+
+    def run(P, iters):
+        stride = max(1, P_full // P)
+        deg = angles_deg(P_full)[::stride][:P]
+        sino = rng.uniform(0.5, 50.0, (deg.size, 1, N)).astype(np.float32)
+        t0 = time.perf_counter()
+        R.run("mlem", deg, sino, N / 2.0, iters, threads)
+        return time.perf_counter() - t0, deg.size
+
+    # size the angle subset so that one iteration takes about target_s
+    t1, p1 = run(max(8, P_full // 100), 1)
+    t2, _ = run(p1, 2)
+    per_angle = max((t2 - t1) / p1, 1e-6)
+    P = int(min(P_full, max(p1, target_s / per_angle)))
+    ta, pa = run(P, 1)
+    tb, _ = run(pa, 2)
+    per_iter_sample = tb - ta  # one iteration, setup cancelled
+    per_iter_full = per_iter_sample * P_full / pa
+    return {
+        "value": 1.0 / per_iter_full,
+        "unit": "slice-iter/s",
+        "cores": threads,
+        "kind": "reference",
+        "sample": (f"reference MLEM engine (oracle/_ref, {threads} threads), one "
+                   f"{N}x{N} slice, {pa} of {P_full} evenly spaced angles, "
+                   f"iteration time = t(2 iters) - t(1 iter) = {per_iter_sample:.2f} s, "
+                   f"scaled x{P_full / pa:.2f} to all angles"),
+        "seconds_per_slice_iter": per_iter_full,
+    }
+
+
+def traffic_record(kernel, workload):
+    """Counter-derived HBM bytes per launch committed under profiles/, or
+    None if no record matches this kernel and workload."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(p) as f:
+            recs = json.load(f)
+    except Exception:
+        return None
+    for r in recs:
+        if r.get("kernel") == kernel and r.get("workload") == workload:
+            return r.get("bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__,
+                                 formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--alg", default="mlem", choices=["mlem", "sirt"])
+    ap.add_argument("--slices", type=int, default=64)
+    ap.add_argument("--size", type=int, default=2048)
+    ap.add_argument("--angles", type=int, default=1500)
+    ap.add_argument("--fast-sqrt", action="store_true",
+                    help="hardware v_sqrt_f32 segment lengths (default: exact)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(slice sharding across GPUs is designed but not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from trace_amd import Projector
+    from trace_amd.phantom import angles_rad, noisy_sinogram
+
+    N, P, S = args.size, args.angles, args.slices
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    workload = (f"{args.alg.upper()}, {S} slices of {N}x{N} fp32, {P} angles in [0,pi), "
+                f"{N} columns, 1 iteration per step")
+    log(f"workload: {workload}")
+
+    theta = angles_rad(P)
+    t0 = time.perf_counter()
+    sino = noisy_sinogram(theta, N, S, seed=2, I0=1e4, K=20, device=dev)
+    torch.cuda.synchronize()
+    log(f"synthetic sinogram {tuple(sino.shape)} in {time.perf_counter() - t0:.1f} s")
+    init = 0.0 if args.alg == "sirt" else 1.0
+    recon = torch.full((S, N, N), init, dtype=torch.float32, device=dev)
+
+    pj = Projector(theta, S, N, center=N / 2.0, device=0, exact_sqrt=not args.fast_sqrt)
+    nnz, oor = pj.count_segments()
+    log(f"segments per slice: {nnz} (outside image: {oor})")
+
+    for w in range(args.warmup):
+        pj.reconstruct(args.alg, 1, sino, recon)
+        torch.cuda.synchronize()
+        log(f"warmup step {w + 1}/{args.warmup} done")
+
+    pj.set_phase_timing(True)
+    pj.phase_times(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        pj.reconstruct(args.alg, 1, sino, recon)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    phases = pj.phase_times(reset=True)
+    pj.set_phase_timing(False)
+    elapsed = t1 - t0
+    log(f"timed {args.steps} steps in {elapsed:.3f} s; phases {phases}")
+
+    finite = bool(torch.isfinite(recon).all().item())
+    nonneg = bool((recon >= 0).all().item()) if args.alg == "mlem" else None
+
+    R = P * N
+    bytes_rays = S * (16.0 * nnz + 12.0 * R + 8.0 * N * N)
+    ms_rays, n_rays = phases["rays"]
+    t_rays = ms_rays / max(n_rays, 1) / 1e3
+    achieved = bytes_rays / t_rays / 1e9
+    kernel = f"k_ray_pass<{args.alg.upper()}>"
+    traffic = traffic_record(kernel, workload)
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"cpu baseline with {threads} threads ...")
+        cpu = cpu_baseline(N, P, threads)
+        if cpu:
+            log(f"cpu baseline: {cpu['sample']}")
+
+    value = S * args.steps / elapsed
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "slice-iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": ("synthetic: 20 seeded ellipses per slice (seed 2), exact line "
+                 "integrals, Poisson noise at 1e4 counts"),
+        "config": {
+            "workload": workload,
+            "alg": args.alg,
+            "slices": S,
+            "image": N,
+            "angles": P,
+            "columns": N,
+            "iters_per_step": 1,
+            "segment_lengths": "v_sqrt_f32" if args.fast_sqrt else "correctly rounded",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": kernel,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": bytes_rays,
+            "avg_launch_ms": t_rays * 1e3,
+            "nnz_per_slice": nnz,
+        },
+        "phases_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in phases.items()},
+        "cpu_baseline": cpu,
+        "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "checks": {"finite": finite, "nonnegative": nonneg},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,180 @@
+/*
+ * trace_hip.h -- C-ABI of the MI355X (gfx950) iterative-reconstruction hot
+ * path.  It replaces, for one process and one GPU, the reference's
+ * reconstruction engine loop:
+ *
+ *   TraceMetadata set-up            src/common/trace_data.h:53-109
+ *   DISPEngineReduction + the       src/common/DISP/core/disp_engine_reduction.h:169-202
+ *   per-algorithm ReconSpace        src/sirt/sirt.h:16-71, src/mlem/mlem.h:16-68,
+ *                                   src/pml/pml.h:63-122, src/apmlr/apmlr.h:55-130
+ *   iteration loop of the mains     src/sirt/sirt_main.cc:177-202,
+ *                                   src/mlem/mlem_main.cc:171-182,
+ *                                   src/pml/pml_main.cc:178-193,
+ *                                   src/apmlr/apmlr_main.cc:209-244
+ *
+ * Conventions
+ * - Plain C types only.  The caller owns every buffer; nothing is freed or
+ *   adopted by the library (the reference's TraceMetadata deletes the theta
+ *   array it is given, trace_data.h:129; this interface copies instead).
+ * - Every entry returns TRACE_OK (0) or a negative TRACE_E_* code; the text
+ *   of the last error of the calling thread is in trace_last_error().
+ * - A handle is bound to one device and one HIP stream; calls on a handle are
+ *   asynchronous on that stream unless stated otherwise, and a handle must
+ *   not be used from two host threads at once.  Distinct handles are
+ *   independent.
+ * - Layouts are the reference's: sinogram [P][S][C] (trace_data.h:139-149,
+ *   trace_h5io.cc:157-205), image [S][N][N] with pixel index ix + iy*N.
+ *   N must equal C (the reference hard-wires num_grids = num_cols,
+ *   sirt.cc:182 / sirt_main.cc:135).
+ * - Theta is in radians.  The reference's per-application preprocessing
+ *   (degree conversion for SIRT/MLEM/APMLR, |sinogram| for MLEM/PML/APMLR,
+ *   initial image 0 for SIRT and 1 otherwise) is the caller's, exactly as in
+ *   the mains; trace_deg2rad() and trace_absolute() reproduce the reference
+ *   helpers bit for bit.
+ */
+#ifndef TRACE_HIP_H_
+#define TRACE_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define TRACE_OK 0
+#define TRACE_E_INVALID (-1)     /* bad argument (null, size, value) */
+#define TRACE_E_UNSUPPORTED (-2) /* unknown algorithm or option */
+#define TRACE_E_HIP (-3)         /* HIP runtime error (text in trace_last_error) */
+#define TRACE_E_NOMEM (-4)       /* device or host allocation failed */
+#define TRACE_E_NONFINITE (-5)   /* NaN or Inf in an input sinogram */
+
+/* Algorithm selection; values follow the draft unified CLI's list
+ * (src/all/trace.cc:112-118). */
+typedef enum {
+  TRACE_SIRT = 0,
+  TRACE_MLEM = 1,
+  TRACE_PML = 2,
+  TRACE_APMLR = 3
+} trace_alg_t;
+
+/* Geometry of one slice stack: the arguments the mains pass to
+ * TraceMetadata (sirt_main.cc:126-136). */
+typedef struct {
+  int num_projs;       /* P: projections (angles) */
+  int num_slices;      /* S: slices */
+  int num_cols;        /* C: detector columns */
+  int num_grids;       /* N: image side, must equal C */
+  float center;        /* rotation centre; <= 0 selects C/2+1 (trace_data.h:97) */
+  const float *theta;  /* [P] projection angles in radians, host memory (copied) */
+} trace_geom_t;
+
+/* Regularisation parameters.  Defaults of the reference CLIs: PML beta 10
+ * (pml_main.cc:48-49); APMLR beta0 10, beta1 1, delta0 1, delta1 1, regw 1
+ * (apmlr_main.cc:22-26). */
+typedef struct {
+  float beta;    /* PML beta, APMLR in-plane beta0 (> 0) */
+  float beta1;   /* APMLR inter-slice beta1 */
+  float delta;   /* APMLR in-plane delta0 (> 0) */
+  float delta1;  /* APMLR inter-slice delta1 (> 0) */
+  float regw;    /* APMLR inter-slice neighbour weight */
+  int apmlr_compat; /* 1: slice s is forward/back-projected with real slice
+                       s-1 (slice 0 with a constant-1 ghost), as the reference
+                       does (apmlr.cc:1082-1084); 0: with slice s */
+} trace_params_t;
+
+typedef struct trace_handle trace_handle_t;
+
+/* Options of a handle (trace_set_option). */
+#define TRACE_OPT_EXACT_SQRT 1 /* 1 (default): correctly rounded segment
+                                  lengths, bit-identical to the reference;
+                                  0: hardware v_sqrt_f32 (<= 1 ulp) */
+#define TRACE_OPT_PHASE_TIMING 2 /* 1: bracket every kernel launch of
+                                    trace_reconstruct with HIP events on the
+                                    handle's stream (trace_phase_times) */
+
+/* Kernel phases reported by trace_phase_times. */
+#define TRACE_PHASE_CLEAR 0  /* zeroing of the accumulators */
+#define TRACE_PHASE_RAYS 1   /* ray pass: A x, weights, A^T w */
+#define TRACE_PHASE_UPDATE 2 /* image update (with the PML/APMLR stencil) */
+#define TRACE_PHASE_SETUP 3  /* column sums, input checks, copies */
+#define TRACE_NUM_PHASES 4
+
+/* Creates a handle for geometry *g on HIP device `device`, working on
+ * `hip_stream` (a hipStream_t; NULL = the device's null stream).  Replaces
+ * the TraceMetadata constructor (trace_data.h:53-109) and the per-row
+ * prologue of Reduce (sirt.cc:111-124): per-angle sinf/cosf/quadrant and
+ * per-column detector coordinates are computed here on the host. */
+int trace_create(trace_handle_t **h, const trace_geom_t *g, int device,
+                 void *hip_stream);
+
+/* Releases every resource of the handle (NULL is accepted). */
+void trace_destroy(trace_handle_t *h);
+
+int trace_set_option(trace_handle_t *h, int option, int value);
+
+/* Runs `iters` iterations of `alg` on device buffers: sino [P][S][C] and
+ * recon [S][N][N] (in: initial image; out: result).  Replaces the mains'
+ * iteration loops (sirt_main.cc:177-202, mlem_main.cc:171-182,
+ * pml_main.cc:178-193, apmlr_main.cc:209-244).  prm may be NULL for SIRT and
+ * MLEM.  Asynchronous except for the input validation, which reads back one
+ * flag (non-finite sinogram values -> TRACE_E_NONFINITE). */
+int trace_reconstruct(trace_handle_t *h, trace_alg_t alg, int iters,
+                      const float *sino, float *recon,
+                      const trace_params_t *prm);
+
+/* As trace_reconstruct with host buffers; copies in and out over PCIe and
+ * returns after the result is in `recon`.  This is the call a reference main
+ * makes in place of its engine loop. */
+int trace_reconstruct_host(trace_handle_t *h, trace_alg_t alg, int iters,
+                           const float *sino, float *recon,
+                           const trace_params_t *prm);
+
+/* y = A x: y [P][S][C] from x [S][N][N] (device buffers).  Per ray the sum
+ * runs in the reference's segment order (CalculateSimdata, sirt.cc:4-19). */
+int trace_forward(trace_handle_t *h, const float *x, float *y);
+
+/* z = A^T r: z [S][N][N] (overwritten) from r [P][S][C] (device buffers).
+ * The scatter half of UpdateReconReplica (sirt.cc:35-64) with a free
+ * per-ray weight. */
+int trace_adjoint(trace_handle_t *h, const float *r, float *z);
+
+/* den = A^T 1 for one slice of this geometry into den [N][N] (device): the
+ * column sums the reference accumulates as the second replica entry
+ * (sirt.cc:62). */
+int trace_column_sums(trace_handle_t *h, float *den);
+
+/* Debug/parity entry: segments of n rays (proj[i], col[i]) (device int
+ * arrays), written to idx/len/len2 [n][max_seg] and counts [n] (device).  The
+ * per-ray pipeline trace_utils.cc:48-187 as executed by the kernels. */
+int trace_dump_rays(trace_handle_t *h, int n, const int *proj, const int *col,
+                    int max_seg, int *counts, int *idx, float *len,
+                    float *len2);
+
+/* Number of segments (nonzeros of A) of one slice, and how many of them fall
+ * outside the image (skipped).  Synchronous. */
+int trace_count_segments(trace_handle_t *h, int64_t *nnz, int64_t *out_of_image);
+
+/* Device time in milliseconds and launch count of each phase, summed over
+ * the launches recorded since the last reset (TRACE_OPT_PHASE_TIMING).
+ * Synchronises with the recorded events.  The reference's -DTIMERON phase
+ * timers (sirt_main.cc:98-101, 174-197) are the counterpart. */
+int trace_phase_times(trace_handle_t *h, double *ms, int64_t *launches, int reset);
+
+/* Host helpers identical to trace_utils::DegreeToRadian
+ * (trace_utils.cc:10-24) and trace_utils::Absolute (trace_utils.cc:4-8). */
+void trace_deg2rad(const float *deg, float *rad, int n);
+void trace_absolute(float *v, int64_t n);
+
+/* Thread-local text of the last error ("" if none). */
+const char *trace_last_error(void);
+
+/* Library version string. */
+const char *trace_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TRACE_HIP_H_ */

@@ -1,0 +1,508 @@
+// capi.hip -- implementation of the C-ABI declared in include/trace_hip.h.
+//
+// The host side does what the reference's TraceMetadata constructor and the
+// per-row prologue of Reduce do (angle tables with the host libm, detector
+// coordinates, centre/mov rules) and drives the per-iteration kernel
+// sequence that replaces DISPEngineReduction + ReconSpace.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/trace_hip.h"
+#include "internal.h"
+
+using namespace trace_hip;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess) {                                                  \
+      return fail(e_ == hipErrorOutOfMemory ? TRACE_E_NOMEM : TRACE_E_HIP,   \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));        \
+    }                                                                        \
+  } while (0)
+
+constexpr float kPI = 3.14159265358979f;  // trace_utils.h:8
+
+// trace_utils.cc:41-46
+int quadrant_of(float q) {
+  return ((q >= 0 && q < kPI / 2) || (q >= kPI && q < 3 * kPI / 2)) ? 1 : 0;
+}
+
+// trace_data.h:97-101: default centre, mov and its unconditional nudge.
+float mov_of(int C, float center) {
+  float c = center;
+  if (c <= 0.) c = (float)((double)(float)C / 2. + 1.);
+  float mov = (float)((double)(float)C / 2. - (double)c);
+  if ((double)(mov - std::ceil(mov)) < 1e-6) mov = (float)((double)mov + 1e-6);
+  return mov;
+}
+
+template <typename T>
+hipError_t dev_alloc(T **p, size_t count) {
+  return hipMalloc((void **)p, count * sizeof(T));
+}
+
+}  // namespace
+
+// HIP-event brackets around kernel launches, per phase.
+struct PhaseTimer {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  double ms[TRACE_NUM_PHASES] = {0, 0, 0, 0};
+  int64_t count[TRACE_NUM_PHASES] = {0, 0, 0, 0};
+  hipEvent_t open_ev = nullptr;
+  int open_phase = -1;
+
+  hipEvent_t take() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+  void begin(int phase, hipStream_t st) {
+    if (!on) return;
+    open_ev = take();
+    open_phase = phase;
+    if (open_ev) (void)hipEventRecord(open_ev, st);
+  }
+  void end(hipStream_t st) {
+    if (!on || !open_ev) return;
+    hipEvent_t e = take();
+    if (!e) return;
+    (void)hipEventRecord(e, st);
+    pending.push_back({open_phase, {open_ev, e}});
+    open_ev = nullptr;
+  }
+  void collect() {
+    for (auto &p : pending) {
+      float t = 0.f;
+      if (hipEventSynchronize(p.second.second) == hipSuccess &&
+          hipEventElapsedTime(&t, p.second.first, p.second.second) == hipSuccess) {
+        ms[p.first] += t;
+        count[p.first] += 1;
+      }
+      pool.push_back(p.second.first);
+      pool.push_back(p.second.second);
+    }
+    pending.clear();
+  }
+  void release() {
+    collect();
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    pool.clear();
+  }
+};
+
+struct trace_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Geometry geo;
+  std::vector<float> theta;
+  bool exact = true;
+  float *den = nullptr;  // [N][N], A^T 1 for one slice
+  bool den_ready = false;
+  bool den_exact = true;
+  float *num = nullptr;  // [S][N][N]
+  float *xa = nullptr;   // [S+2][N][N] work image (PML/APMLR ping-pong)
+  float *xb = nullptr;   // [S+2][N][N]
+  int *flag = nullptr;   // validation flag
+  unsigned long long *counters = nullptr;  // [2]
+  PhaseTimer timer;
+};
+
+// Launch bracketed by the phase timer.
+#define TIMED(phase, expr)                  \
+  do {                                      \
+    h->timer.begin((phase), h->stream);     \
+    HIP_TRY(expr);                          \
+    h->timer.end(h->stream);                \
+  } while (0)
+
+namespace {
+
+int ensure_den(trace_handle *h) {
+  if (h->den_ready && h->den_exact == h->exact) return TRACE_OK;
+  TIMED(TRACE_PHASE_SETUP, launch_fill(h->den, h->geo.NN(), 0.f, h->stream));
+  TIMED(TRACE_PHASE_SETUP, launch_den(h->geo, h->den, h->stream, h->exact));
+  h->den_ready = true;
+  h->den_exact = h->exact;
+  return TRACE_OK;
+}
+
+int ensure_work(trace_handle *h, bool images) {
+  const size_t nn = (size_t)h->geo.NN();
+  const size_t S = (size_t)h->geo.S;
+  if (!h->num) HIP_TRY(dev_alloc(&h->num, S * nn));
+  if (images) {
+    if (!h->xa) HIP_TRY(dev_alloc(&h->xa, (S + 2) * nn));
+    if (!h->xb) HIP_TRY(dev_alloc(&h->xb, (S + 2) * nn));
+  }
+  return TRACE_OK;
+}
+
+// APMLR weights (apmlr.cc:40-51, 94-101, 245-251 and the one-sided variants
+// in FGTop/FGBottom): float total from a double expression, axial 1/total in
+// float, diagonal (1/sqrt 2)/total in double, z regw/total in float.
+void apmlr_weights(float regw, StencilWeights &w) {
+  static const int nin[9] = {8, 5, 5, 5, 5, 3, 3, 3, 3};
+  static const int naxial[9] = {4, 3, 3, 3, 3, 2, 2, 2, 2};
+  std::memset(&w, 0, sizeof(w));
+  for (int zm = 0; zm < 3; ++zm) {
+    const int nz = zm == 0 ? 2 : 1;
+    for (int cls = 0; cls < 9; ++cls) {
+      const double base = cls == 0 ? 4 + 4 / std::sqrt(2.)
+                                   : (cls <= 4 ? 3 + 2 / std::sqrt(2.) : 2 + 1 / std::sqrt(2.));
+      const double zterm = nz == 2 ? (double)(2 * regw) : (double)regw;
+      const float total = (float)(base + zterm);
+      for (int q = 0; q < nin[cls]; ++q)
+        w.w[zm][cls][q] = q < naxial[cls] ? 1 / total : (float)(1 / std::sqrt(2.) / total);
+      for (int q = 0; q < nz; ++q) w.w[zm][cls][nin[cls] + q] = regw / total;
+    }
+  }
+}
+
+int check_finite(trace_handle *h, const float *v, int64_t n) {
+  HIP_TRY(hipMemsetAsync(h->flag, 0, sizeof(int), h->stream));
+  HIP_TRY(launch_check_finite(v, n, h->flag, h->stream));
+  int flag = 0;
+  HIP_TRY(hipMemcpyAsync(&flag, h->flag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (flag) return fail(TRACE_E_NONFINITE, "sinogram holds NaN or Inf values");
+  return TRACE_OK;
+}
+
+int validate_params(trace_alg_t alg, const trace_params_t *prm) {
+  if (alg == TRACE_PML || alg == TRACE_APMLR) {
+    if (!prm) return fail(TRACE_E_INVALID, "PML/APMLR need parameters");
+    if (!(prm->beta > 0.f) || !std::isfinite(prm->beta))
+      return fail(TRACE_E_INVALID, "beta must be finite and > 0 (F = 0 otherwise)");
+    if (alg == TRACE_APMLR) {
+      if (!(prm->delta > 0.f) || !(prm->delta1 > 0.f) || !std::isfinite(prm->delta) ||
+          !std::isfinite(prm->delta1))
+        return fail(TRACE_E_INVALID, "delta and delta1 must be finite and > 0");
+      if (!std::isfinite(prm->beta1) || !std::isfinite(prm->regw) || prm->beta1 < 0.f ||
+          prm->regw < 0.f)
+        return fail(TRACE_E_INVALID, "beta1 and regw must be finite and >= 0");
+      if (prm->apmlr_compat != 0 && prm->apmlr_compat != 1)
+        return fail(TRACE_E_INVALID, "apmlr_compat must be 0 or 1");
+    }
+  }
+  return TRACE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *trace_last_error(void) { return g_err.c_str(); }
+
+const char *trace_version(void) { return "trace-hip 0.1 (gfx950)"; }
+
+void trace_deg2rad(const float *deg, float *rad, int n) {
+  // trace_utils.cc:22: float product, double division, float result
+  if (!deg || !rad) return;
+  for (int i = 0; i < n; ++i) rad[i] = (float)((double)(deg[i] * kPI) / 180.0);
+}
+
+void trace_absolute(float *v, int64_t n) {
+  if (!v) return;
+  for (int64_t i = 0; i < n; ++i) v[i] = std::fabs(v[i]);
+}
+
+int trace_create(trace_handle_t **out, const trace_geom_t *g, int device,
+                 void *hip_stream) {
+  g_err.clear();
+  if (!out) return fail(TRACE_E_INVALID, "handle pointer is null");
+  *out = nullptr;
+  if (!g) return fail(TRACE_E_INVALID, "geometry is null");
+  if (!g->theta) return fail(TRACE_E_INVALID, "theta is null");
+  if (g->num_projs <= 0 || g->num_slices <= 0 || g->num_cols <= 1 || g->num_grids <= 1)
+    return fail(TRACE_E_INVALID, "P, S must be >= 1 and C, N >= 2");
+  if (g->num_grids != g->num_cols)
+    return fail(TRACE_E_INVALID, "num_grids must equal num_cols (sirt.cc:182)");
+  if (g->num_cols > 32768) return fail(TRACE_E_INVALID, "num_cols above 32768");
+  if (!std::isfinite(g->center)) return fail(TRACE_E_INVALID, "center is not finite");
+  if (device < 0) return fail(TRACE_E_INVALID, "device index is negative");
+  for (int p = 0; p < g->num_projs; ++p)
+    if (!std::isfinite(g->theta[p])) return fail(TRACE_E_INVALID, "theta holds NaN/Inf");
+
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device >= ndev) return fail(TRACE_E_INVALID, "device index out of range");
+  HIP_TRY(hipSetDevice(device));
+
+  auto *h = new trace_handle();
+  h->device = device;
+  h->stream = (hipStream_t)hip_stream;
+  Geometry &geo = h->geo;
+  geo.P = g->num_projs;
+  geo.S = g->num_slices;
+  geo.C = g->num_cols;
+  geo.N = g->num_grids;
+  geo.center = g->center;
+  geo.mov = mov_of(geo.C, g->center);
+  geo.grid.N = geo.N;
+  geo.grid.C = geo.C;
+  geo.grid.gx0 = (float)(-(double)geo.N / 2.);
+  geo.grid.gxN = (float)(-(double)geo.N / 2. + geo.N);
+  geo.grid.mg = (float)((double)geo.N / 2.);
+  h->theta.assign(g->theta, g->theta + geo.P);
+
+  std::vector<AngleInfo> ang(geo.P);
+  for (int p = 0; p < geo.P; ++p) {
+    const float q = h->theta[p];
+    ang[p].sinq = sinf(q);  // host libm, as sirt.cc:119-120
+    ang[p].cosq = cosf(q);
+    ang[p].quadrant = quadrant_of(q);
+    ang[p].pad = 0;
+  }
+  std::vector<float> yi(geo.C);
+  for (int c = 0; c < geo.C; ++c)  // sirt.cc:129
+    yi[c] = (float)((double)(1 - geo.C) / 2. + (double)c + (double)geo.mov);
+
+  auto cleanup_fail = [&](int code) {
+    trace_destroy(h);
+    return code;
+  };
+  hipError_t e;
+  if ((e = dev_alloc(&geo.d_ang, (size_t)geo.P)) != hipSuccess ||
+      (e = dev_alloc(&geo.d_yi, (size_t)geo.C)) != hipSuccess ||
+      (e = dev_alloc(&h->den, (size_t)geo.NN())) != hipSuccess ||
+      (e = dev_alloc(&h->flag, 1)) != hipSuccess ||
+      (e = dev_alloc(&h->counters, 2)) != hipSuccess)
+    return cleanup_fail(fail(e == hipErrorOutOfMemory ? TRACE_E_NOMEM : TRACE_E_HIP,
+                             std::string("allocation: ") + hipGetErrorString(e)));
+  if ((e = hipMemcpy(geo.d_ang, ang.data(), ang.size() * sizeof(AngleInfo),
+                     hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(geo.d_yi, yi.data(), yi.size() * sizeof(float),
+                     hipMemcpyHostToDevice)) != hipSuccess)
+    return cleanup_fail(fail(TRACE_E_HIP, std::string("upload: ") + hipGetErrorString(e)));
+  *out = h;
+  return TRACE_OK;
+}
+
+void trace_destroy(trace_handle_t *h) {
+  if (!h) return;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  else (void)hipDeviceSynchronize();
+  h->timer.release();
+  for (void *p : {(void *)h->geo.d_ang, (void *)h->geo.d_yi, (void *)h->den,
+                  (void *)h->num, (void *)h->xa, (void *)h->xb, (void *)h->flag,
+                  (void *)h->counters})
+    (void)hipFree(p);
+  delete h;
+}
+
+int trace_set_option(trace_handle_t *h, int option, int value) {
+  g_err.clear();
+  if (!h) return fail(TRACE_E_INVALID, "handle is null");
+  if (value != 0 && value != 1) return fail(TRACE_E_INVALID, "value must be 0 or 1");
+  if (option == TRACE_OPT_EXACT_SQRT) {
+    h->exact = value != 0;
+    return TRACE_OK;
+  }
+  if (option == TRACE_OPT_PHASE_TIMING) {
+    h->timer.on = value != 0;
+    return TRACE_OK;
+  }
+  return fail(TRACE_E_UNSUPPORTED, "unknown option");
+}
+
+int trace_reconstruct(trace_handle_t *h, trace_alg_t alg, int iters,
+                      const float *sino, float *recon, const trace_params_t *prm) {
+  g_err.clear();
+  if (!h) return fail(TRACE_E_INVALID, "handle is null");
+  if (!sino || !recon) return fail(TRACE_E_INVALID, "null buffer");
+  if (alg < TRACE_SIRT || alg > TRACE_APMLR) return fail(TRACE_E_UNSUPPORTED, "unknown algorithm");
+  if (iters < 0) return fail(TRACE_E_INVALID, "iters must be >= 0");
+  if (int rc = validate_params(alg, prm)) return rc;
+  HIP_TRY(hipSetDevice(h->device));
+  const Geometry &g = h->geo;
+  const int64_t NN = g.NN();
+  const int S = g.S;
+  if (int rc = check_finite(h, sino, (int64_t)g.P * S * g.C)) return rc;
+  if (iters == 0) return TRACE_OK;
+  if (int rc = ensure_den(h)) return rc;
+  const bool reg = (alg == TRACE_PML || alg == TRACE_APMLR);
+  if (int rc = ensure_work(h, reg)) return rc;
+  hipStream_t st = h->stream;
+
+  if (alg == TRACE_SIRT || alg == TRACE_MLEM) {
+    for (int it = 0; it < iters; ++it) {
+      TIMED(TRACE_PHASE_CLEAR, launch_fill(h->num, S * NN, 0.f, st));
+      TIMED(TRACE_PHASE_RAYS, launch_ray_pass(alg, g, recon, NN, sino, h->num, st, h->exact));
+      if (alg == TRACE_SIRT)
+        TIMED(TRACE_PHASE_UPDATE, launch_update_sirt(recon, h->num, h->den, S, NN, st));
+      else
+        TIMED(TRACE_PHASE_UPDATE, launch_update_mlem(recon, h->num, h->den, S, NN, st));
+    }
+    return TRACE_OK;
+  }
+  if (alg == TRACE_PML) {
+    float *cur = recon, *alt = h->xa;
+    for (int it = 0; it < iters; ++it) {
+      TIMED(TRACE_PHASE_CLEAR, launch_fill(h->num, S * NN, 0.f, st));
+      TIMED(TRACE_PHASE_RAYS, launch_ray_pass(kPML, g, cur, NN, sino, h->num, st, h->exact));
+      TIMED(TRACE_PHASE_UPDATE, launch_update_pml(cur, alt, h->num, h->den, S, g.N, prm->beta,
+                                                  st, nullptr, nullptr));
+      std::swap(cur, alt);
+    }
+    if (cur != recon)
+      HIP_TRY(hipMemcpyAsync(recon, cur, sizeof(float) * S * NN, hipMemcpyDeviceToDevice, st));
+    return TRACE_OK;
+  }
+  // APMLR on ghost-padded images [S+2][N][N]; ghosts hold the reference's
+  // initial value 1 (apmlr_main.cc:170) and are never updated at one rank.
+  StencilWeights w;
+  apmlr_weights(prm->regw, w);
+  RegParams rp{prm->beta, prm->beta1, prm->delta, prm->delta1, prm->regw};
+  float *pa = h->xa, *pb = h->xb;
+  HIP_TRY(launch_fill(pa, (S + 2) * NN, 1.f, st));
+  HIP_TRY(launch_fill(pb, (S + 2) * NN, 1.f, st));
+  HIP_TRY(hipMemcpyAsync(pa + NN, recon, sizeof(float) * S * NN, hipMemcpyDeviceToDevice, st));
+  for (int it = 0; it < iters; ++it) {
+    TIMED(TRACE_PHASE_CLEAR, launch_fill(h->num, S * NN, 0.f, st));
+    const float *xpass = prm->apmlr_compat ? pa : pa + NN;
+    TIMED(TRACE_PHASE_RAYS, launch_ray_pass(kAPMLR, g, xpass, NN, sino, h->num, st, h->exact));
+    TIMED(TRACE_PHASE_UPDATE,
+          launch_update_apmlr(pa, pb, h->num, h->den, S, g.N, rp, w, st, nullptr, nullptr));
+    std::swap(pa, pb);
+  }
+  HIP_TRY(hipMemcpyAsync(recon, pa + NN, sizeof(float) * S * NN, hipMemcpyDeviceToDevice, st));
+  return TRACE_OK;
+}
+
+int trace_reconstruct_host(trace_handle_t *h, trace_alg_t alg, int iters,
+                           const float *sino, float *recon, const trace_params_t *prm) {
+  g_err.clear();
+  if (!h) return fail(TRACE_E_INVALID, "handle is null");
+  if (!sino || !recon) return fail(TRACE_E_INVALID, "null buffer");
+  HIP_TRY(hipSetDevice(h->device));
+  const Geometry &g = h->geo;
+  const size_t nsino = (size_t)g.P * g.S * g.C, nrec = (size_t)g.S * g.NN();
+  float *ds = nullptr, *dr = nullptr;
+  hipError_t e = dev_alloc(&ds, nsino);
+  if (e == hipSuccess) e = dev_alloc(&dr, nrec);
+  if (e != hipSuccess) {
+    (void)hipFree(ds);
+    return fail(e == hipErrorOutOfMemory ? TRACE_E_NOMEM : TRACE_E_HIP,
+                std::string("allocation: ") + hipGetErrorString(e));
+  }
+  int rc = TRACE_OK;
+  if ((e = hipMemcpyAsync(ds, sino, nsino * 4, hipMemcpyHostToDevice, h->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(dr, recon, nrec * 4, hipMemcpyHostToDevice, h->stream)) != hipSuccess)
+    rc = fail(TRACE_E_HIP, std::string("upload: ") + hipGetErrorString(e));
+  if (rc == TRACE_OK) rc = trace_reconstruct(h, alg, iters, ds, dr, prm);
+  if (rc == TRACE_OK) {
+    if ((e = hipMemcpyAsync(recon, dr, nrec * 4, hipMemcpyDeviceToHost, h->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(h->stream)) != hipSuccess)
+      rc = fail(TRACE_E_HIP, std::string("download: ") + hipGetErrorString(e));
+  } else {
+    (void)hipStreamSynchronize(h->stream);
+  }
+  (void)hipFree(ds);
+  (void)hipFree(dr);
+  return rc;
+}
+
+int trace_forward(trace_handle_t *h, const float *x, float *y) {
+  g_err.clear();
+  if (!h) return fail(TRACE_E_INVALID, "handle is null");
+  if (!x || !y) return fail(TRACE_E_INVALID, "null buffer");
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(launch_forward(h->geo, x, h->geo.NN(), y, h->stream, h->exact));
+  return TRACE_OK;
+}
+
+int trace_adjoint(trace_handle_t *h, const float *r, float *z) {
+  g_err.clear();
+  if (!h) return fail(TRACE_E_INVALID, "handle is null");
+  if (!r || !z) return fail(TRACE_E_INVALID, "null buffer");
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(launch_fill(z, (int64_t)h->geo.S * h->geo.NN(), 0.f, h->stream));
+  HIP_TRY(launch_adjoint(h->geo, r, z, h->stream, h->exact));
+  return TRACE_OK;
+}
+
+int trace_column_sums(trace_handle_t *h, float *den) {
+  g_err.clear();
+  if (!h) return fail(TRACE_E_INVALID, "handle is null");
+  if (!den) return fail(TRACE_E_INVALID, "null buffer");
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(launch_fill(den, h->geo.NN(), 0.f, h->stream));
+  HIP_TRY(launch_den(h->geo, den, h->stream, h->exact));
+  return TRACE_OK;
+}
+
+int trace_dump_rays(trace_handle_t *h, int n, const int *proj, const int *col,
+                    int max_seg, int *counts, int *idx, float *len, float *len2) {
+  g_err.clear();
+  if (!h) return fail(TRACE_E_INVALID, "handle is null");
+  if (n < 0 || max_seg < 0) return fail(TRACE_E_INVALID, "negative size");
+  if (n == 0) return TRACE_OK;
+  if (!proj || !col || !counts || !idx || !len || !len2)
+    return fail(TRACE_E_INVALID, "null buffer");
+  HIP_TRY(hipSetDevice(h->device));
+  // the ray ids index device tables: check them on the host first
+  std::vector<int> hp(n), hc(n);
+  HIP_TRY(hipMemcpy(hp.data(), proj, n * sizeof(int), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(hc.data(), col, n * sizeof(int), hipMemcpyDeviceToHost));
+  for (int i = 0; i < n; ++i)
+    if (hp[i] < 0 || hp[i] >= h->geo.P || hc[i] < 0 || hc[i] >= h->geo.C)
+      return fail(TRACE_E_INVALID, "ray id out of range");
+  HIP_TRY(launch_dump_rays(h->geo, n, proj, col, max_seg, counts, idx, len, len2,
+                           h->stream, h->exact));
+  return TRACE_OK;
+}
+
+int trace_phase_times(trace_handle_t *h, double *ms, int64_t *launches, int reset) {
+  g_err.clear();
+  if (!h) return fail(TRACE_E_INVALID, "handle is null");
+  HIP_TRY(hipSetDevice(h->device));
+  h->timer.collect();
+  for (int i = 0; i < TRACE_NUM_PHASES; ++i) {
+    if (ms) ms[i] = h->timer.ms[i];
+    if (launches) launches[i] = h->timer.count[i];
+    if (reset) {
+      h->timer.ms[i] = 0.0;
+      h->timer.count[i] = 0;
+    }
+  }
+  return TRACE_OK;
+}
+
+int trace_count_segments(trace_handle_t *h, int64_t *nnz, int64_t *out_of_image) {
+  g_err.clear();
+  if (!h || !nnz) return fail(TRACE_E_INVALID, "null argument");
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipMemsetAsync(h->counters, 0, 2 * sizeof(unsigned long long), h->stream));
+  HIP_TRY(launch_count_segments(h->geo, h->counters, h->counters + 1, h->stream));
+  unsigned long long c[2] = {0, 0};
+  HIP_TRY(hipMemcpyAsync(c, h->counters, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  *nnz = (int64_t)c[0];
+  if (out_of_image) *out_of_image = (int64_t)c[1];
+  return TRACE_OK;
+}
+
+}  // extern "C"
