@@ -11,11 +11,16 @@ A step is one full iteration over the whole stack (zero accumulators, the
 ray pass A x -> weights -> A^T w, image update).  `value` is slice-iterations
 per second (= iterations per second per 2048^2 slice, summed over slices).
 
-`roofline` reports the dominant kernel (the ray pass): algorithmic bytes per
-launch = slices x (16 nnz + 12 R + 8 N^2) where nnz is the exact segment
-count of one slice (counted by the traversal) and R = P x C rays, divided by
-the kernel's average launch time measured with HIP events on the launch
-stream; peak 8 TB/s.
+`roofline` reports the dominant kernel of the step.  Algorithmic bytes per
+slice-iteration follow the stored-operator model B = 16 nnz + 12 R + 24 N^2
+(nnz = exact segment count of one slice, counted by the traversal; R = P x C
+rays), split over the three kernels of an iteration:
+  forward projector  8 nnz + 8 R + 4 N^2  (A once, read p, write w, read x)
+  backprojector      8 nnz + 4 R + 4 N^2  (A once, read w, write z)
+  image update       16 N^2               (read x, z, den; write x)
+`achieved` = the dominant kernel's bytes per launch (x slices) / its
+average launch time, measured with HIP events on the launch stream; peak
+8 TB/s.  The kernels regenerate A on the fly, so `frac` can exceed 1.
 
 `cpu_baseline` times the reference's own threaded CPU engine (compiled from
 its sources into oracle/_ref) on this host: one MLEM iteration of one
@@ -171,12 +176,25 @@ def main():
     nonneg = bool((recon >= 0).all().item()) if args.alg == "mlem" else None
 
     R = P * N
-    bytes_rays = S * (16.0 * nnz + 12.0 * R + 8.0 * N * N)
-    ms_rays, n_rays = phases["rays"]
-    t_rays = ms_rays / max(n_rays, 1) / 1e3
-    achieved = bytes_rays / t_rays / 1e9
-    kernel = f"k_ray_pass<{args.alg.upper()}>"
+    model = {
+        "forward": ("k_forward", S * (8.0 * nnz + 8.0 * R + 4.0 * N * N)),
+        "back": ("k_bp_tile", S * (8.0 * nnz + 4.0 * R + 4.0 * N * N)),
+        "update": ("k_update_em", S * 16.0 * N * N),
+    }
+    per_kernel = {}
+    for ph, (kname, nbytes) in model.items():
+        ms, n = phases[ph]
+        t = ms / max(n, 1) / 1e3
+        per_kernel[ph] = {"kernel": kname, "avg_launch_ms": t * 1e3, "launches": n,
+                          "algorithmic_bytes_per_launch": nbytes,
+                          "achieved_GBs": nbytes / t / 1e9 if t > 0 else None}
+    dom = max(("forward", "back"), key=lambda k: per_kernel[k]["avg_launch_ms"])
+    kernel = per_kernel[dom]["kernel"]
+    t_dom = per_kernel[dom]["avg_launch_ms"] / 1e3
+    bytes_dom = per_kernel[dom]["algorithmic_bytes_per_launch"]
+    achieved = bytes_dom / t_dom / 1e9
     traffic = traffic_record(kernel, workload)
+    b_alg_step = S * (16.0 * nnz + 12.0 * R + 24.0 * N * N)
 
     cpu = None
     if not args.no_cpu:
@@ -219,10 +237,13 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "algorithmic_bytes_per_launch": bytes_rays,
-            "avg_launch_ms": t_rays * 1e3,
+            "algorithmic_bytes_per_launch": bytes_dom,
+            "avg_launch_ms": t_dom * 1e3,
             "nnz_per_slice": nnz,
+            "step_model_frac": b_alg_step / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS,
         },
+        "kernels": per_kernel,
+        "launch_shape": list(pj.launch_shape()),
         "phases_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in phases.items()},
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,

@@ -90,15 +90,18 @@ typedef struct trace_handle trace_handle_t;
 #define TRACE_OPT_EXACT_SQRT 1 /* 1 (default): correctly rounded segment
                                   lengths, bit-identical to the reference;
                                   0: hardware v_sqrt_f32 (<= 1 ulp) */
-#define TRACE_OPT_PHASE_TIMING 2 /* 1: bracket every kernel launch of
-                                    trace_reconstruct with HIP events on the
-                                    handle's stream (trace_phase_times) */
+#define TRACE_OPT_PHASE_TIMING 2 /* 1: bracket every kernel launch with HIP
+                                    events on the handle's stream
+                                    (trace_phase_times) */
+#define TRACE_OPT_BACKPROJECTOR 3 /* 0 (default): LDS-tiled backprojector;
+                                     1: whole-ray scatter with global float
+                                     atomics (an independent check) */
 
 /* Kernel phases reported by trace_phase_times. */
-#define TRACE_PHASE_CLEAR 0  /* zeroing of the accumulators */
-#define TRACE_PHASE_RAYS 1   /* ray pass: A x, weights, A^T w */
-#define TRACE_PHASE_UPDATE 2 /* image update (with the PML/APMLR stencil) */
-#define TRACE_PHASE_SETUP 3  /* column sums, input checks, copies */
+#define TRACE_PHASE_FORWARD 0 /* forward projection and per-ray weights */
+#define TRACE_PHASE_BACK 1    /* backprojection A^T w */
+#define TRACE_PHASE_UPDATE 2  /* image update (with the PML/APMLR stencil) */
+#define TRACE_PHASE_SETUP 3   /* ray table, column sums, packing, checks */
 #define TRACE_NUM_PHASES 4
 
 /* Creates a handle for geometry *g on HIP device `device`, working on
@@ -113,6 +116,17 @@ int trace_create(trace_handle_t **h, const trace_geom_t *g, int device,
 void trace_destroy(trace_handle_t *h);
 
 int trace_set_option(trace_handle_t *h, int option, int value);
+
+/* Launch shape: slices per forward-projector lane (4, 8 or 16) and the
+ * backprojector's image tile (tile_w x tile_h pixels, tile_batch slices per
+ * workgroup).  Only the shapes compiled into the library are accepted
+ * (TRACE_E_UNSUPPORTED otherwise); trace_amd.autotune picks among them.
+ * No reference counterpart (the reference's work unit is fixed at one
+ * sinogram row, sirt_main.cc:172). */
+int trace_set_launch_shape(trace_handle_t *h, int fwd_batch, int tile_w, int tile_h,
+                           int tile_batch);
+int trace_get_launch_shape(trace_handle_t *h, int *fwd_batch, int *tile_w, int *tile_h,
+                           int *tile_batch);
 
 /* Runs `iters` iterations of `alg` on device buffers: sino [P][S][C] and
  * recon [S][N][N] (in: initial image; out: result).  Replaces the mains'
@@ -144,6 +158,11 @@ int trace_adjoint(trace_handle_t *h, const float *r, float *z);
  * column sums the reference accumulates as the second replica entry
  * (sirt.cc:62). */
 int trace_column_sums(trace_handle_t *h, float *den);
+
+/* Parity entry: number of segments per pixel of one slice, counts [N][N]
+ * (device, float), through the configured backprojector.  Integer-exact, so
+ * the tile and whole-ray backprojectors must agree bit for bit. */
+int trace_segment_counts(trace_handle_t *h, float *counts);
 
 /* Debug/parity entry: segments of n rays (proj[i], col[i]) (device int
  * arrays), written to idx/len/len2 [n][max_seg] and counts [n] (device).  The

@@ -132,3 +132,25 @@ def test_nonfinite_sinogram_rejected(gpu):
     with Projector(th, 1, N, N / 2) as pj:
         with pytest.raises(TraceNonFinite):
             pj.reconstruct("mlem", 1, sino, x)
+
+
+@pytest.mark.parametrize("shape", [(4, 64, 64, 4), (8, 32, 32, 8), (16, 32, 32, 16),
+                                   (8, 64, 64, 8)])
+@pytest.mark.parametrize("alg", ["sirt", "mlem"])
+def test_launch_shapes_recon(gpu, oracle, alg, shape):
+    """Every compiled launch shape gives the same reconstruction."""
+    import torch
+    from trace_amd import Projector
+    from trace_amd.phantom import angles_rad, noisy_sinogram
+    N, P, S, iters = 72, 50, 9, 4
+    th = angles_rad(P)
+    sino = noisy_sinogram(th, N, S, seed=7, I0=1e4, K=6).numpy()
+    init = np.full((S, N, N), 0.0 if alg == "sirt" else 1.0, np.float32)
+    with Projector(th, S, N, N / 2) as pj:
+        pj.set_launch_shape(*shape)
+        dx = torch.from_numpy(init).cuda()
+        pj.reconstruct(alg, iters, torch.from_numpy(sino).cuda(), dx)
+        torch.cuda.synchronize()
+        x = dx.cpu().numpy()
+    xo = oracle.reconstruct(alg, th, sino, init, N / 2, iters)
+    assert rel_l2(x, xo) < TOL, shape

@@ -25,7 +25,8 @@ ALGS = {"sirt": TRACE_SIRT, "mlem": TRACE_MLEM, "pml": TRACE_PML, "apmlr": TRACE
 
 TRACE_OPT_EXACT_SQRT = 1
 TRACE_OPT_PHASE_TIMING = 2
-PHASES = ("clear", "rays", "update", "setup")
+TRACE_OPT_BACKPROJECTOR = 3
+PHASES = ("forward", "back", "update", "setup")
 
 
 class TraceGeom(ct.Structure):
@@ -86,6 +87,10 @@ _SIGS = {
     "trace_create": (ct.c_int, [ct.POINTER(_vp), ct.POINTER(TraceGeom), ct.c_int, _vp]),
     "trace_destroy": (None, [_vp]),
     "trace_set_option": (ct.c_int, [_vp, ct.c_int, ct.c_int]),
+    "trace_set_launch_shape": (ct.c_int, [_vp, ct.c_int, ct.c_int, ct.c_int, ct.c_int]),
+    "trace_get_launch_shape": (ct.c_int, [_vp, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int),
+                                          ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),
+    "trace_segment_counts": (ct.c_int, [_vp, _vp]),
     "trace_reconstruct": (ct.c_int, [_vp, ct.c_int, ct.c_int, _vp, _vp,
                                      ct.POINTER(TraceParams)]),
     "trace_reconstruct_host": (ct.c_int, [_vp, ct.c_int, ct.c_int, _vp, _vp,

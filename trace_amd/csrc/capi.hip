@@ -3,13 +3,18 @@
 // The host side does what the reference's TraceMetadata constructor and the
 // per-row prologue of Reduce do (angle tables with the host libm, detector
 // coordinates, centre/mov rules) and drives the per-iteration kernel
-// sequence that replaces DISPEngineReduction + ReconSpace.
+// sequence that replaces DISPEngineReduction + ReconSpace:
+//
+//   forward projector   y = A x, per-ray weight w        (k_forward)
+//   tile backprojector  z = A^T w                        (k_bp_tile)
+//   image update        x' from x, z, den [, stencil]    (k_update_*)
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/trace_hip.h"
@@ -56,7 +61,14 @@ hipError_t dev_alloc(T **p, size_t count) {
   return hipMalloc((void **)p, count * sizeof(T));
 }
 
-}  // namespace
+// Launch shapes compiled into the library (kernels.hip, launch_backproject).
+bool tile_shape_supported(int tw, int th, int sb) {
+  static const int shapes[][3] = {{64, 64, 4}, {32, 32, 4}, {64, 32, 4}, {32, 64, 4},
+                                  {32, 32, 8}, {64, 32, 8}, {64, 64, 8}, {32, 32, 16}};
+  for (auto &s : shapes)
+    if (s[0] == tw && s[1] == th && s[2] == sb) return true;
+  return false;
+}
 
 // HIP-event brackets around kernel launches, per phase.
 struct PhaseTimer {
@@ -112,49 +124,110 @@ struct PhaseTimer {
   }
 };
 
+}  // namespace
+
 struct trace_handle {
   int device = 0;
   hipStream_t stream = nullptr;
   Geometry geo;
   std::vector<float> theta;
+  std::vector<AngleInfo> ang;
   bool exact = true;
-  float *den = nullptr;  // [N][N], A^T 1 for one slice
+  int backprojector = 0;  // 0 tile (LDS), 1 whole-ray global atomics
+  int fwd_sb = 4;         // slices per forward lane (xb layout)
+  TileShape tile;
+  RayInfo *rays = nullptr;     // [P][C]
+  bool rays_ready = false;
+  int *count = nullptr;        // [P] columns-per-tile bound for count_shape
+  TileShape count_shape{0, 0, 0};
+  float *den = nullptr;        // [N][N]
   bool den_ready = false;
   bool den_exact = true;
-  float *num = nullptr;  // [S][N][N]
-  float *xa = nullptr;   // [S+2][N][N] work image (PML/APMLR ping-pong)
-  float *xb = nullptr;   // [S+2][N][N]
-  int *flag = nullptr;   // validation flag
+  float *xb = nullptr;         // [ceil(S/fwd_sb)][N][N][fwd_sb]
+  int xb_sb = 0;
+  float *w = nullptr;          // [P][S][C]
+  float *z = nullptr;          // [S][N][N]
+  float *xa = nullptr;         // [S+2][N][N] work image (PML/APMLR)
+  float *xc = nullptr;         // [S+2][N][N]
+  int *flag = nullptr;
   unsigned long long *counters = nullptr;  // [2]
   PhaseTimer timer;
 };
 
 // Launch bracketed by the phase timer.
-#define TIMED(phase, expr)                  \
-  do {                                      \
-    h->timer.begin((phase), h->stream);     \
-    HIP_TRY(expr);                          \
-    h->timer.end(h->stream);                \
+#define TIMED(phase, expr)              \
+  do {                                  \
+    h->timer.begin((phase), h->stream); \
+    HIP_TRY(expr);                      \
+    h->timer.end(h->stream);            \
   } while (0)
 
 namespace {
 
+int ensure_rays(trace_handle *h) {
+  if (h->rays_ready) return TRACE_OK;
+  if (!h->rays) HIP_TRY(dev_alloc(&h->rays, (size_t)h->geo.rays_per_slice()));
+  TIMED(TRACE_PHASE_SETUP, launch_ray_info(h->geo, h->rays, h->stream));
+  h->rays_ready = true;
+  return TRACE_OK;
+}
+
+int ensure_count(trace_handle *h) {
+  const TileShape &t = h->tile;
+  if (h->count && h->count_shape.tw == t.tw && h->count_shape.th == t.th) return TRACE_OK;
+  std::vector<int> cnt(h->geo.P);
+  for (int p = 0; p < h->geo.P; ++p)
+    cnt[p] = tile_count_bound(h->ang[p].sinq, h->ang[p].cosq, t.tw, t.th);
+  if (!h->count) HIP_TRY(dev_alloc(&h->count, (size_t)h->geo.P));
+  HIP_TRY(hipMemcpyAsync(h->count, cnt.data(), cnt.size() * sizeof(int), hipMemcpyHostToDevice,
+                         h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));  // cnt leaves scope
+  h->count_shape = t;
+  return TRACE_OK;
+}
+
+// A^T applied with the configured backprojector.  mode: kBpOutSino (w
+// [P][S][C] -> z [S][N][N]), kBpOutOnes / kBpOutCount (-> z [N][N]).
+int backproject(trace_handle *h, int mode, const float *w, float *z, int phase) {
+  if (int rc = ensure_rays(h)) return rc;
+  if (h->backprojector == 0) {
+    if (int rc = ensure_count(h)) return rc;
+    TIMED(phase, launch_backproject(h->geo, h->rays, h->count, h->tile, mode, h->exact, w, z,
+                                    h->stream));
+  } else {
+    const int ns = (mode == kBpOutSino) ? h->geo.S : 1;
+    TIMED(phase, launch_fill(z, (int64_t)ns * h->geo.NN(), 0.f, h->stream));
+    TIMED(phase, launch_adjoint_atomic(h->geo, h->rays, mode, h->exact, w, z, ns, h->stream));
+  }
+  return TRACE_OK;
+}
+
 int ensure_den(trace_handle *h) {
   if (h->den_ready && h->den_exact == h->exact) return TRACE_OK;
-  TIMED(TRACE_PHASE_SETUP, launch_fill(h->den, h->geo.NN(), 0.f, h->stream));
-  TIMED(TRACE_PHASE_SETUP, launch_den(h->geo, h->den, h->stream, h->exact));
+  if (int rc = backproject(h, kBpOutOnes, nullptr, h->den, TRACE_PHASE_SETUP)) return rc;
   h->den_ready = true;
   h->den_exact = h->exact;
   return TRACE_OK;
 }
 
+int nbatch(int S, int sb) { return (S + sb - 1) / sb; }
+
 int ensure_work(trace_handle *h, bool images) {
   const size_t nn = (size_t)h->geo.NN();
   const size_t S = (size_t)h->geo.S;
-  if (!h->num) HIP_TRY(dev_alloc(&h->num, S * nn));
+  if (h->xb && h->xb_sb != h->fwd_sb) {
+    (void)hipFree(h->xb);
+    h->xb = nullptr;
+  }
+  if (!h->xb) {
+    HIP_TRY(dev_alloc(&h->xb, (size_t)nbatch((int)S, h->fwd_sb) * h->fwd_sb * nn));
+    h->xb_sb = h->fwd_sb;
+  }
+  if (!h->w) HIP_TRY(dev_alloc(&h->w, (size_t)h->geo.P * S * h->geo.C));
+  if (!h->z) HIP_TRY(dev_alloc(&h->z, S * nn));
   if (images) {
     if (!h->xa) HIP_TRY(dev_alloc(&h->xa, (S + 2) * nn));
-    if (!h->xb) HIP_TRY(dev_alloc(&h->xb, (S + 2) * nn));
+    if (!h->xc) HIP_TRY(dev_alloc(&h->xc, (S + 2) * nn));
   }
   return TRACE_OK;
 }
@@ -215,7 +288,7 @@ extern "C" {
 
 const char *trace_last_error(void) { return g_err.c_str(); }
 
-const char *trace_version(void) { return "trace-hip 0.1 (gfx950)"; }
+const char *trace_version(void) { return "trace-hip 0.2 (gfx950)"; }
 
 void trace_deg2rad(const float *deg, float *rad, int n) {
   // trace_utils.cc:22: float product, double division, float result
@@ -260,6 +333,7 @@ int trace_create(trace_handle_t **out, const trace_geom_t *g, int device,
   geo.N = g->num_grids;
   geo.center = g->center;
   geo.mov = mov_of(geo.C, g->center);
+  geo.yi0 = (float)((double)(1 - geo.C) / 2. + (double)geo.mov);
   geo.grid.N = geo.N;
   geo.grid.C = geo.C;
   geo.grid.gx0 = (float)(-(double)geo.N / 2.);
@@ -267,13 +341,13 @@ int trace_create(trace_handle_t **out, const trace_geom_t *g, int device,
   geo.grid.mg = (float)((double)geo.N / 2.);
   h->theta.assign(g->theta, g->theta + geo.P);
 
-  std::vector<AngleInfo> ang(geo.P);
+  h->ang.resize(geo.P);
   for (int p = 0; p < geo.P; ++p) {
     const float q = h->theta[p];
-    ang[p].sinq = sinf(q);  // host libm, as sirt.cc:119-120
-    ang[p].cosq = cosf(q);
-    ang[p].quadrant = quadrant_of(q);
-    ang[p].pad = 0;
+    h->ang[p].sinq = sinf(q);  // host libm, as sirt.cc:119-120
+    h->ang[p].cosq = cosf(q);
+    h->ang[p].quadrant = quadrant_of(q);
+    h->ang[p].pad = 0;
   }
   std::vector<float> yi(geo.C);
   for (int c = 0; c < geo.C; ++c)  // sirt.cc:129
@@ -291,7 +365,7 @@ int trace_create(trace_handle_t **out, const trace_geom_t *g, int device,
       (e = dev_alloc(&h->counters, 2)) != hipSuccess)
     return cleanup_fail(fail(e == hipErrorOutOfMemory ? TRACE_E_NOMEM : TRACE_E_HIP,
                              std::string("allocation: ") + hipGetErrorString(e)));
-  if ((e = hipMemcpy(geo.d_ang, ang.data(), ang.size() * sizeof(AngleInfo),
+  if ((e = hipMemcpy(geo.d_ang, h->ang.data(), h->ang.size() * sizeof(AngleInfo),
                      hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(geo.d_yi, yi.data(), yi.size() * sizeof(float),
                      hipMemcpyHostToDevice)) != hipSuccess)
@@ -306,9 +380,9 @@ void trace_destroy(trace_handle_t *h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   else (void)hipDeviceSynchronize();
   h->timer.release();
-  for (void *p : {(void *)h->geo.d_ang, (void *)h->geo.d_yi, (void *)h->den,
-                  (void *)h->num, (void *)h->xa, (void *)h->xb, (void *)h->flag,
-                  (void *)h->counters})
+  for (void *p : {(void *)h->geo.d_ang, (void *)h->geo.d_yi, (void *)h->rays, (void *)h->count,
+                  (void *)h->den, (void *)h->xb, (void *)h->w, (void *)h->z, (void *)h->xa,
+                  (void *)h->xc, (void *)h->flag, (void *)h->counters})
     (void)hipFree(p);
   delete h;
 }
@@ -316,16 +390,46 @@ void trace_destroy(trace_handle_t *h) {
 int trace_set_option(trace_handle_t *h, int option, int value) {
   g_err.clear();
   if (!h) return fail(TRACE_E_INVALID, "handle is null");
-  if (value != 0 && value != 1) return fail(TRACE_E_INVALID, "value must be 0 or 1");
-  if (option == TRACE_OPT_EXACT_SQRT) {
-    h->exact = value != 0;
-    return TRACE_OK;
+  switch (option) {
+    case TRACE_OPT_EXACT_SQRT:
+      if (value != 0 && value != 1) return fail(TRACE_E_INVALID, "value must be 0 or 1");
+      h->exact = value != 0;
+      return TRACE_OK;
+    case TRACE_OPT_PHASE_TIMING:
+      if (value != 0 && value != 1) return fail(TRACE_E_INVALID, "value must be 0 or 1");
+      h->timer.on = value != 0;
+      return TRACE_OK;
+    case TRACE_OPT_BACKPROJECTOR:
+      if (value != 0 && value != 1) return fail(TRACE_E_INVALID, "value must be 0 or 1");
+      h->backprojector = value;
+      return TRACE_OK;
+    default:
+      return fail(TRACE_E_UNSUPPORTED, "unknown option");
   }
-  if (option == TRACE_OPT_PHASE_TIMING) {
-    h->timer.on = value != 0;
-    return TRACE_OK;
-  }
-  return fail(TRACE_E_UNSUPPORTED, "unknown option");
+}
+
+int trace_set_launch_shape(trace_handle_t *h, int fwd_batch, int tile_w, int tile_h,
+                           int tile_batch) {
+  g_err.clear();
+  if (!h) return fail(TRACE_E_INVALID, "handle is null");
+  if (fwd_batch != 4 && fwd_batch != 8 && fwd_batch != 16)
+    return fail(TRACE_E_UNSUPPORTED, "forward slice batch must be 4, 8 or 16");
+  if (!tile_shape_supported(tile_w, tile_h, tile_batch))
+    return fail(TRACE_E_UNSUPPORTED, "tile shape not compiled into this library");
+  h->fwd_sb = fwd_batch;
+  h->tile = TileShape{tile_w, tile_h, tile_batch};
+  return TRACE_OK;
+}
+
+int trace_get_launch_shape(trace_handle_t *h, int *fwd_batch, int *tile_w, int *tile_h,
+                           int *tile_batch) {
+  g_err.clear();
+  if (!h) return fail(TRACE_E_INVALID, "handle is null");
+  if (fwd_batch) *fwd_batch = h->fwd_sb;
+  if (tile_w) *tile_w = h->tile.tw;
+  if (tile_h) *tile_h = h->tile.th;
+  if (tile_batch) *tile_batch = h->tile.sb;
+  return TRACE_OK;
 }
 
 int trace_reconstruct(trace_handle_t *h, trace_alg_t alg, int iters,
@@ -342,29 +446,36 @@ int trace_reconstruct(trace_handle_t *h, trace_alg_t alg, int iters,
   const int S = g.S;
   if (int rc = check_finite(h, sino, (int64_t)g.P * S * g.C)) return rc;
   if (iters == 0) return TRACE_OK;
+  if (int rc = ensure_rays(h)) return rc;
   if (int rc = ensure_den(h)) return rc;
   const bool reg = (alg == TRACE_PML || alg == TRACE_APMLR);
   if (int rc = ensure_work(h, reg)) return rc;
   hipStream_t st = h->stream;
+  const int sb = h->fwd_sb;
+  const int nslots = nbatch(S, sb) * sb;
+  const int fmode = (alg == TRACE_SIRT) ? kFwdOutSIRT : kFwdOutEM;
 
   if (alg == TRACE_SIRT || alg == TRACE_MLEM) {
+    TIMED(TRACE_PHASE_SETUP, launch_pack(recon, S, 0, 0.f, NN, nslots, sb, h->xb, st));
     for (int it = 0; it < iters; ++it) {
-      TIMED(TRACE_PHASE_CLEAR, launch_fill(h->num, S * NN, 0.f, st));
-      TIMED(TRACE_PHASE_RAYS, launch_ray_pass(alg, g, recon, NN, sino, h->num, st, h->exact));
-      if (alg == TRACE_SIRT)
-        TIMED(TRACE_PHASE_UPDATE, launch_update_sirt(recon, h->num, h->den, S, NN, st));
-      else
-        TIMED(TRACE_PHASE_UPDATE, launch_update_mlem(recon, h->num, h->den, S, NN, st));
+      TIMED(TRACE_PHASE_FORWARD,
+            launch_forward(g, h->rays, sb, fmode, h->exact, h->xb, sino, h->w, st));
+      if (int rc = backproject(h, kBpOutSino, h->w, h->z, TRACE_PHASE_BACK)) return rc;
+      TIMED(TRACE_PHASE_UPDATE,
+            launch_update_em(alg == TRACE_SIRT, recon, h->z, h->den, S, NN, sb, h->xb, st));
     }
     return TRACE_OK;
   }
   if (alg == TRACE_PML) {
     float *cur = recon, *alt = h->xa;
+    TIMED(TRACE_PHASE_SETUP, launch_pack(cur, S, 0, 0.f, NN, nslots, sb, h->xb, st));
     for (int it = 0; it < iters; ++it) {
-      TIMED(TRACE_PHASE_CLEAR, launch_fill(h->num, S * NN, 0.f, st));
-      TIMED(TRACE_PHASE_RAYS, launch_ray_pass(kPML, g, cur, NN, sino, h->num, st, h->exact));
-      TIMED(TRACE_PHASE_UPDATE, launch_update_pml(cur, alt, h->num, h->den, S, g.N, prm->beta,
+      TIMED(TRACE_PHASE_FORWARD,
+            launch_forward(g, h->rays, sb, fmode, h->exact, h->xb, sino, h->w, st));
+      if (int rc = backproject(h, kBpOutSino, h->w, h->z, TRACE_PHASE_BACK)) return rc;
+      TIMED(TRACE_PHASE_UPDATE, launch_update_pml(cur, alt, h->z, h->den, S, g.N, prm->beta,
                                                   st, nullptr, nullptr));
+      TIMED(TRACE_PHASE_UPDATE, launch_pack(alt, S, 0, 0.f, NN, nslots, sb, h->xb, st));
       std::swap(cur, alt);
     }
     if (cur != recon)
@@ -373,19 +484,23 @@ int trace_reconstruct(trace_handle_t *h, trace_alg_t alg, int iters,
   }
   // APMLR on ghost-padded images [S+2][N][N]; ghosts hold the reference's
   // initial value 1 (apmlr_main.cc:170) and are never updated at one rank.
+  // The ray pass of slice s reads padded slice s (compat) or s+1.
   StencilWeights w;
   apmlr_weights(prm->regw, w);
   RegParams rp{prm->beta, prm->beta1, prm->delta, prm->delta1, prm->regw};
-  float *pa = h->xa, *pb = h->xb;
-  HIP_TRY(launch_fill(pa, (S + 2) * NN, 1.f, st));
-  HIP_TRY(launch_fill(pb, (S + 2) * NN, 1.f, st));
+  const int off = prm->apmlr_compat ? 0 : 1;
+  float *pa = h->xa, *pb = h->xc;
+  TIMED(TRACE_PHASE_SETUP, launch_fill(pa, (S + 2) * NN, 1.f, st));
+  TIMED(TRACE_PHASE_SETUP, launch_fill(pb, (S + 2) * NN, 1.f, st));
   HIP_TRY(hipMemcpyAsync(pa + NN, recon, sizeof(float) * S * NN, hipMemcpyDeviceToDevice, st));
+  TIMED(TRACE_PHASE_SETUP, launch_pack(pa, S + 2, off, 0.f, NN, nslots, sb, h->xb, st));
   for (int it = 0; it < iters; ++it) {
-    TIMED(TRACE_PHASE_CLEAR, launch_fill(h->num, S * NN, 0.f, st));
-    const float *xpass = prm->apmlr_compat ? pa : pa + NN;
-    TIMED(TRACE_PHASE_RAYS, launch_ray_pass(kAPMLR, g, xpass, NN, sino, h->num, st, h->exact));
-    TIMED(TRACE_PHASE_UPDATE,
-          launch_update_apmlr(pa, pb, h->num, h->den, S, g.N, rp, w, st, nullptr, nullptr));
+    TIMED(TRACE_PHASE_FORWARD,
+          launch_forward(g, h->rays, sb, kFwdOutEM, h->exact, h->xb, sino, h->w, st));
+    if (int rc = backproject(h, kBpOutSino, h->w, h->z, TRACE_PHASE_BACK)) return rc;
+    TIMED(TRACE_PHASE_UPDATE, launch_update_apmlr(pa, pb, h->z, h->den, S, g.N, rp, w,
+                                                  prm->apmlr_compat, st, nullptr, nullptr));
+    TIMED(TRACE_PHASE_UPDATE, launch_pack(pb, S + 2, off, 0.f, NN, nslots, sb, h->xb, st));
     std::swap(pa, pb);
   }
   HIP_TRY(hipMemcpyAsync(recon, pa + NN, sizeof(float) * S * NN, hipMemcpyDeviceToDevice, st));
@@ -430,7 +545,14 @@ int trace_forward(trace_handle_t *h, const float *x, float *y) {
   if (!h) return fail(TRACE_E_INVALID, "handle is null");
   if (!x || !y) return fail(TRACE_E_INVALID, "null buffer");
   HIP_TRY(hipSetDevice(h->device));
-  HIP_TRY(launch_forward(h->geo, x, h->geo.NN(), y, h->stream, h->exact));
+  if (int rc = ensure_rays(h)) return rc;
+  if (int rc = ensure_work(h, false)) return rc;
+  const Geometry &g = h->geo;
+  const int sb = h->fwd_sb;
+  TIMED(TRACE_PHASE_SETUP,
+        launch_pack(x, g.S, 0, 0.f, g.NN(), nbatch(g.S, sb) * sb, sb, h->xb, h->stream));
+  TIMED(TRACE_PHASE_FORWARD,
+        launch_forward(g, h->rays, sb, kFwdOutY, h->exact, h->xb, nullptr, y, h->stream));
   return TRACE_OK;
 }
 
@@ -439,9 +561,7 @@ int trace_adjoint(trace_handle_t *h, const float *r, float *z) {
   if (!h) return fail(TRACE_E_INVALID, "handle is null");
   if (!r || !z) return fail(TRACE_E_INVALID, "null buffer");
   HIP_TRY(hipSetDevice(h->device));
-  HIP_TRY(launch_fill(z, (int64_t)h->geo.S * h->geo.NN(), 0.f, h->stream));
-  HIP_TRY(launch_adjoint(h->geo, r, z, h->stream, h->exact));
-  return TRACE_OK;
+  return backproject(h, kBpOutSino, r, z, TRACE_PHASE_BACK);
 }
 
 int trace_column_sums(trace_handle_t *h, float *den) {
@@ -449,9 +569,15 @@ int trace_column_sums(trace_handle_t *h, float *den) {
   if (!h) return fail(TRACE_E_INVALID, "handle is null");
   if (!den) return fail(TRACE_E_INVALID, "null buffer");
   HIP_TRY(hipSetDevice(h->device));
-  HIP_TRY(launch_fill(den, h->geo.NN(), 0.f, h->stream));
-  HIP_TRY(launch_den(h->geo, den, h->stream, h->exact));
-  return TRACE_OK;
+  return backproject(h, kBpOutOnes, nullptr, den, TRACE_PHASE_SETUP);
+}
+
+int trace_segment_counts(trace_handle_t *h, float *counts) {
+  g_err.clear();
+  if (!h) return fail(TRACE_E_INVALID, "handle is null");
+  if (!counts) return fail(TRACE_E_INVALID, "null buffer");
+  HIP_TRY(hipSetDevice(h->device));
+  return backproject(h, kBpOutCount, nullptr, counts, TRACE_PHASE_SETUP);
 }
 
 int trace_dump_rays(trace_handle_t *h, int n, const int *proj, const int *col,
@@ -470,7 +596,8 @@ int trace_dump_rays(trace_handle_t *h, int n, const int *proj, const int *col,
   for (int i = 0; i < n; ++i)
     if (hp[i] < 0 || hp[i] >= h->geo.P || hc[i] < 0 || hc[i] >= h->geo.C)
       return fail(TRACE_E_INVALID, "ray id out of range");
-  HIP_TRY(launch_dump_rays(h->geo, n, proj, col, max_seg, counts, idx, len, len2,
+  if (int rc = ensure_rays(h)) return rc;
+  HIP_TRY(launch_dump_rays(h->geo, h->rays, n, proj, col, max_seg, counts, idx, len, len2,
                            h->stream, h->exact));
   return TRACE_OK;
 }
@@ -495,8 +622,9 @@ int trace_count_segments(trace_handle_t *h, int64_t *nnz, int64_t *out_of_image)
   g_err.clear();
   if (!h || !nnz) return fail(TRACE_E_INVALID, "null argument");
   HIP_TRY(hipSetDevice(h->device));
+  if (int rc = ensure_rays(h)) return rc;
   HIP_TRY(hipMemsetAsync(h->counters, 0, 2 * sizeof(unsigned long long), h->stream));
-  HIP_TRY(launch_count_segments(h->geo, h->counters, h->counters + 1, h->stream));
+  HIP_TRY(launch_count_segments(h->geo, h->rays, h->counters, h->counters + 1, h->stream));
   unsigned long long c[2] = {0, 0};
   HIP_TRY(hipMemcpyAsync(c, h->counters, sizeof(c), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));

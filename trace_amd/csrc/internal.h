@@ -4,17 +4,22 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+
 #include "traverse.h"
 
 namespace trace_hip {
 
 enum Alg { kSIRT = 0, kMLEM = 1, kPML = 2, kAPMLR = 3 };
+enum FwdModeHost { kFwdOutY = 0, kFwdOutSIRT = 1, kFwdOutEM = 2 };
+enum BpModeHost { kBpOutSino = 0, kBpOutOnes = 1, kBpOutCount = 2 };
 
 // One slice stack's geometry on the device.
 struct Geometry {
   int P = 0, S = 0, C = 0, N = 0;
   float center = 0.f;  // as given by the caller
   float mov = 0.f;     // trace_data.h:97-101
+  float yi0 = 0.f;     // detector coordinate of column 0 (sirt.cc:129)
   GridInfo grid{};
   AngleInfo *d_ang = nullptr;  // [P]
   float *d_yi = nullptr;       // [C]
@@ -22,13 +27,19 @@ struct Geometry {
   int64_t rays_per_slice() const { return (int64_t)P * C; }
 };
 
+// Launch shape of the tile backprojector: tile of tw x th pixels, sb slices
+// per workgroup (LDS: tw*th*sb floats).
+struct TileShape {
+  int tw = 64, th = 64, sb = 4;
+};
+
 // Regularisation weights for the PML/APMLR stencils, computed on the host
 // with the reference's float/double promotion order.
 struct StencilWeights {
   // [z mode][pixel class][neighbour]: z mode 0 = two z neighbours,
-  // 1 = first slice (+1 only), 2 = last slice (-1 only); classes as in
-  // traverse-free stencil code (0 inner, 1 top, 2 bottom, 3 left, 4 right,
-  // 5..8 corners TL, TR, BL, BR); neighbours in-plane first, then z.
+  // 1 = first slice (+1 only), 2 = last slice (-1 only); classes 0 inner,
+  // 1 top, 2 bottom, 3 left, 4 right, 5..8 corners TL, TR, BL, BR;
+  // neighbours in-plane first, then z.
   float w[3][9][10];
 };
 
@@ -36,34 +47,38 @@ struct RegParams {
   float beta, beta1, delta, delta1, regw;
 };
 
+// Upper bound of the detector columns whose ray can touch one tile at an
+// angle (host side, for the flattened (angle, column) work list).
+int tile_count_bound(float sinq, float cosq, int TW, int TH);
+
 // Launchers (kernels.hip).  All are asynchronous on `st`.
-hipError_t launch_den(const Geometry &g, float *den, hipStream_t st, bool exact);
-hipError_t launch_forward(const Geometry &g, const float *x, int64_t x_stride,
-                          float *y, hipStream_t st, bool exact);
-hipError_t launch_adjoint(const Geometry &g, const float *r, float *z,
-                          hipStream_t st, bool exact);
-hipError_t launch_ray_pass(int alg, const Geometry &g, const float *x,
-                           int64_t x_stride, const float *sino, float *num,
-                           hipStream_t st, bool exact);
+hipError_t launch_ray_info(const Geometry &g, RayInfo *rays, hipStream_t st);
+hipError_t launch_forward(const Geometry &g, const RayInfo *rays, int sb, int mode,
+                          bool exact, const float *xb, const float *sino, float *out,
+                          hipStream_t st);
+hipError_t launch_backproject(const Geometry &g, const RayInfo *rays, const int *count,
+                              const TileShape &ts, int mode, bool exact, const float *w,
+                              float *z, hipStream_t st);
+hipError_t launch_pack(const float *src, int nsrc, int off, float fill, int64_t NN,
+                       int nslots, int sb, float *xb, hipStream_t st);
+hipError_t launch_update_em(bool sirt, float *x, const float *z, const float *den, int S,
+                            int64_t NN, int sb, float *xb, hipStream_t st);
 hipError_t launch_fill(float *p, int64_t n, float v, hipStream_t st);
-hipError_t launch_update_sirt(float *x, const float *num, const float *den,
-                              int S, int64_t NN, hipStream_t st);
-hipError_t launch_update_mlem(float *x, const float *num, const float *den,
-                              int S, int64_t NN, hipStream_t st);
-hipError_t launch_update_pml(const float *x, float *xn, const float *num,
-                             const float *den, int S, int N, float beta,
-                             hipStream_t st, float *F_out, float *G_out);
-hipError_t launch_update_apmlr(const float *xpad, float *xnpad,
-                               const float *num, const float *den, int S,
-                               int N, const RegParams &p,
-                               const StencilWeights &w, hipStream_t st,
+hipError_t launch_update_pml(const float *x, float *xn, const float *z, const float *den,
+                             int S, int N, float beta, hipStream_t st, float *F_out,
+                             float *G_out);
+hipError_t launch_update_apmlr(const float *xpad, float *xnpad, const float *z,
+                               const float *den, int S, int N, const RegParams &p,
+                               const StencilWeights &w, int compat, hipStream_t st,
                                float *F_out, float *G_out);
-hipError_t launch_dump_rays(const Geometry &g, int n, const int *proj,
-                            const int *col, int max_seg, int *counts, int *idx,
-                            float *len, float *len2, hipStream_t st, bool exact);
-hipError_t launch_count_segments(const Geometry &g, unsigned long long *total,
-                                 unsigned long long *oor, hipStream_t st);
-hipError_t launch_check_finite(const float *v, int64_t n, int *flag,
-                               hipStream_t st);
+hipError_t launch_adjoint_atomic(const Geometry &g, const RayInfo *rays, int mode, bool exact,
+                                 const float *r, float *z, int nslices, hipStream_t st);
+hipError_t launch_dump_rays(const Geometry &g, const RayInfo *rays, int n, const int *proj,
+                            const int *col, int max_seg, int *counts, int *idx, float *len,
+                            float *len2, hipStream_t st, bool exact);
+hipError_t launch_count_segments(const Geometry &g, const RayInfo *rays,
+                                 unsigned long long *total, unsigned long long *oor,
+                                 hipStream_t st);
+hipError_t launch_check_finite(const float *v, int64_t n, int *flag, hipStream_t st);
 
 }  // namespace trace_hip

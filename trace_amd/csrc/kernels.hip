@@ -1,12 +1,28 @@
 // kernels.hip -- gfx950 kernels of the reconstruction hot path.
 //
-// Ray kernels: one lane per ray; the 64 lanes of a wavefront are 64 adjacent
-// detector columns of one (projection, slice) row, so at every step they
-// touch neighbouring pixels and their gathers/atomics share cache lines.
-// Segments are generated on the fly by traverse_ray (traverse.h).
+// Data layout in HBM (per handle):
+//   x      [S][N][N]        image, the reference's layout (idx = ix + iy*N)
+//   xb     [S/SB][N*N][SB]  slice-batched copy of x read by the forward
+//                           projector: one segment gathers SB slices as
+//                           contiguous float4s
+//   w      [P][S][C]        per-ray weights written by the forward projector
+//   z      [S][N][N]        A^T w written by the tile backprojector
+//   den    [N][N]           A^T 1 (iteration invariant, one slice's geometry)
+//   rays   [P][C]           RayInfo: source point, slopes, trimmed ranges
 //
-// Image kernels: one lane per pixel, rows of pixels across a wavefront for
-// coalesced HBM traffic.
+// Forward projector: one lane per ray (p, c) and slice batch; the 64 lanes
+// of a wavefront are adjacent detector columns of one angle, so at every
+// step they gather from neighbouring pixels.  Each lane walks the whole ray
+// in the reference's order and accumulates y = sum x[idx]*len per slice, so
+// y is bit-identical to CalculateSimdata (sirt.cc:4-19).
+//
+// Backprojector: one workgroup per (image tile, slice batch) owns the tile's
+// accumulators in LDS for all angles.  Its lanes take the (angle, column)
+// pairs whose ray can touch the tile, walk only the part of the ray near the
+// tile (traverse_window) and add len*w into LDS with ds_add_f32; the tile
+// is then written to HBM with coalesced stores.  No global atomics, no
+// per-thread image replicas (the reference's DISP replicas + tree merge,
+// disp_engine_reduction.h:104-185).
 #include <hip/hip_runtime.h>
 
 #include "internal.h"
@@ -19,136 +35,254 @@ namespace {
 
 constexpr int kRayBlock = 256;
 constexpr int kPixBlock = 256;
+constexpr int kTileBlock = 256;
 
 __device__ __forceinline__ void atomic_add_f32(float *p, float v) {
-  // hardware global float atomic (no CAS loop); relaxed ordering, device scope
-  unsafeAtomicAdd(p, v);
+  unsafeAtomicAdd(p, v);  // hardware global float atomic, relaxed
 }
 
-// ray id t -> (p, s, c) in sinogram order [P][S][C]
-__device__ __forceinline__ void ray_coords(int64_t t, int S, int C, int &p,
-                                           int &s, int &c) {
-  c = (int)(t % C);
-  int64_t ps = t / C;
-  s = (int)(ps % S);
-  p = (int)(ps / S);
-}
+// ---------------------------------------------------------------- rays --
 
-template <bool kExact>
 __global__ void __launch_bounds__(kRayBlock)
-k_den(GridInfo g, const AngleInfo *__restrict__ ang, const float *__restrict__ yi,
-      int P, float *__restrict__ den) {
+k_ray_info(GridInfo g, const AngleInfo *__restrict__ ang, const float *__restrict__ yi,
+           int P, RayInfo *__restrict__ rays) {
+  const int64_t t = (int64_t)blockIdx.x * kRayBlock + threadIdx.x;
+  if (t >= (int64_t)P * g.C) return;
+  const int c = (int)(t % g.C), p = (int)(t / g.C);
+  rays[t] = make_ray(g, yi[c], ang[p].sinq, ang[p].cosq);
+}
+
+// ------------------------------------------------------------- forward --
+
+enum FwdMode { kFwdY = 0, kFwdSIRT = 1, kFwdEM = 2 };
+
+template <int SB>
+struct Vec {
+  float v[SB];
+};
+
+template <int SB>
+__device__ __forceinline__ void load_slices(const float *__restrict__ p, float (&o)[SB]) {
+  static_assert(SB % 4 == 0, "slice batch must be a multiple of 4");
+#pragma unroll
+  for (int k = 0; k < SB; k += 4) {
+    const float4 q = *reinterpret_cast<const float4 *>(p + k);
+    o[k] = q.x; o[k + 1] = q.y; o[k + 2] = q.z; o[k + 3] = q.w;
+  }
+}
+
+// One lane per ray (p, c) and slice batch b (blockIdx.y).  kFwdY writes
+// y[p][s][c]; kFwdSIRT / kFwdEM write the per-ray weight of UpdateReconReplica
+//   SIRT  w = (p - y)/a2, a2 = sum len^2    (sirt.cc:47-52)
+//   EM    w = p/y                           (mlem.cc:50, pml.cc:236)
+template <int SB, int MODE, bool kExact>
+__global__ void __launch_bounds__(kRayBlock)
+k_forward(GridInfo g, const AngleInfo *__restrict__ ang, const RayInfo *__restrict__ rays,
+          int P, int S, const float *__restrict__ xb, const float *__restrict__ sino,
+          float *__restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * kRayBlock + threadIdx.x;
   const int C = g.C;
   if (t >= (int64_t)P * C) return;
   const int c = (int)(t % C), p = (int)(t / C);
+  const int b = blockIdx.y;
   const unsigned NN = (unsigned)g.N * (unsigned)g.N;
-  traverse_ray<kExact>(g, yi[c], ang[p], [&](int idx, float l, float) {
-    if ((unsigned)idx < NN) atomic_add_f32(den + idx, l);
-  });
-}
-
-// y[p][s][c] = sum_k x_s[idx_k] * len_k in segment order (sirt.cc:4-19).
-template <bool kExact>
-__global__ void __launch_bounds__(kRayBlock)
-k_forward(GridInfo g, const AngleInfo *__restrict__ ang,
-          const float *__restrict__ yi, int P, int S,
-          const float *__restrict__ x, int64_t x_stride, float *__restrict__ y) {
-  const int64_t t = (int64_t)blockIdx.x * kRayBlock + threadIdx.x;
-  const int C = g.C;
-  if (t >= (int64_t)P * S * C) return;
-  int p, s, c;
-  ray_coords(t, S, C, p, s, c);
-  const float *xs = x + (int64_t)s * x_stride;
-  const unsigned NN = (unsigned)g.N * (unsigned)g.N;
-  float acc = 0.f;
-  traverse_ray<kExact>(g, yi[c], ang[p], [&](int idx, float l, float) {
-    if ((unsigned)idx < NN) acc += xs[idx] * l;
-  });
-  y[t] = acc;
-}
-
-// z_s[idx] += len * r[p][s][c] (the adjoint; z must be zeroed).
-template <bool kExact>
-__global__ void __launch_bounds__(kRayBlock)
-k_adjoint(GridInfo g, const AngleInfo *__restrict__ ang,
-          const float *__restrict__ yi, int P, int S,
-          const float *__restrict__ r, float *__restrict__ z) {
-  const int64_t t = (int64_t)blockIdx.x * kRayBlock + threadIdx.x;
-  const int C = g.C;
-  if (t >= (int64_t)P * S * C) return;
-  int p, s, c;
-  ray_coords(t, S, C, p, s, c);
-  const unsigned NN = (unsigned)g.N * (unsigned)g.N;
-  float *zs = z + (int64_t)s * NN;
-  const float w = r[t];
-  if (w == 0.f) return;
-  traverse_ray<kExact>(g, yi[c], ang[p], [&](int idx, float l, float) {
-    if ((unsigned)idx < NN) atomic_add_f32(zs + idx, l * w);
-  });
-}
-
-// One iteration's ray work for one ray (Reduce + UpdateReconReplica):
-// simdata (and SIRT's a2) in a first walk, the per-ray weight, then the
-// scatter of the numerator in a second walk.  den is iteration-invariant and
-// precomputed by k_den.
-//   SIRT  w = (p - sim)/a2;  num += len*w               (sirt.cc:35-64)
-//   MLEM  w = p/sim;         num += len*w               (mlem.cc:37-61)
-//   PML/APMLR w = p/sim;     num -= x*len*w             (pml.cc:222-246)
-template <int ALG, bool kExact>
-__global__ void __launch_bounds__(kRayBlock)
-k_ray_pass(GridInfo g, const AngleInfo *__restrict__ ang,
-           const float *__restrict__ yi, int P, int S,
-           const float *__restrict__ x, int64_t x_stride,
-           const float *__restrict__ sino, float *__restrict__ num) {
-  const int64_t t = (int64_t)blockIdx.x * kRayBlock + threadIdx.x;
-  const int C = g.C;
-  if (t >= (int64_t)P * S * C) return;
-  int p, s, c;
-  ray_coords(t, S, C, p, s, c);
-  const float *xs = x + (int64_t)s * x_stride;
-  const unsigned NN = (unsigned)g.N * (unsigned)g.N;
-  float *nums = num + (int64_t)s * NN;
-  const float yic = yi[c];
-  const AngleInfo a = ang[p];
-  float sim = 0.f, a2 = 0.f;
-  traverse_ray<kExact>(g, yic, a, [&](int idx, float l, float l2) {
-    if ((unsigned)idx < NN) sim += xs[idx] * l;
-    if (ALG == kSIRT) a2 += l2;
-  });
-  const float ray = sino[t];
-  float w;
-  if (ALG == kSIRT)
-    w = (ray - sim) / a2;
-  else
-    w = ray / sim;
-  traverse_ray<kExact>(g, yic, a, [&](int idx, float l, float) {
-    if ((unsigned)idx < NN) {
-      if (ALG == kSIRT || ALG == kMLEM)
-        atomic_add_f32(nums + idx, l * w);
-      else
-        atomic_add_f32(nums + idx, -(xs[idx] * l * w));
+  const float *xbb = xb + (int64_t)b * NN * SB;
+  float acc[SB];
+#pragma unroll
+  for (int k = 0; k < SB; ++k) acc[k] = 0.f;
+  float a2 = 0.f;
+  const RayInfo r = rays[t];
+  traverse_full<kExact>(g, r, ang[p].quadrant, [&](int ix, int iy, float l, float l2) {
+    const unsigned idx = (unsigned)(ix + iy * g.N);
+    if (idx < NN) {
+      float v[SB];
+      load_slices<SB>(xbb + (int64_t)idx * SB, v);
+#pragma unroll
+      for (int k = 0; k < SB; ++k) acc[k] += v[k] * l;
     }
+    if (MODE == kFwdSIRT) a2 += l2;
   });
+#pragma unroll
+  for (int k = 0; k < SB; ++k) {
+    const int s = b * SB + k;
+    if (s >= S) break;
+    const int64_t o = ((int64_t)p * S + s) * C + c;
+    if (MODE == kFwdY) {
+      out[o] = acc[k];
+    } else if (MODE == kFwdSIRT) {
+      out[o] = (sino[o] - acc[k]) / a2;
+    } else {
+      out[o] = sino[o] / acc[k];
+    }
+  }
+}
+
+// ------------------------------------------------------- backprojector --
+
+enum BpMode { kBpSino = 0, kBpOnes = 1, kBpCount = 2 };
+
+struct TileArgs {
+  GridInfo g;
+  const AngleInfo *ang;
+  const RayInfo *rays;
+  const int *count;   // [P] upper bound of the columns touching any tile
+  int P, S;
+  float yi0;          // detector coordinate of column 0
+  const float *w;     // [P][S][C] weights (kBpSino)
+  float *z;           // [S][N][N] output (kBpOnes/kBpCount: [N][N])
+};
+
+// Columns whose ray can touch tile [x0,x1]x[y0,y1] at angle p: the ray of
+// column c is the line q.(-sin, cos) = yi0 + c; its merged points lie within
+// 0.1 pixel of that line, so a 1-pixel margin on the tile's projection is
+// ample.
+__device__ __forceinline__ void tile_columns(const AngleInfo &a, float x0, float x1, float y0,
+                                             float y1, float yi0, int &cmin, int &cmax) {
+  const float s = a.sinq, c = a.cosq;
+  const float t00 = -x0 * s + y0 * c, t01 = -x0 * s + y1 * c;
+  const float t10 = -x1 * s + y0 * c, t11 = -x1 * s + y1 * c;
+  const float tmin = fminf(fminf(t00, t01), fminf(t10, t11));
+  const float tmax = fmaxf(fmaxf(t00, t01), fmaxf(t10, t11));
+  cmin = (int)ceilf(tmin - 1.0f - yi0);
+  cmax = (int)floorf(tmax + 1.0f - yi0);
+}
+
+// Workgroup (tile, slice batch): LDS accumulators acc[TH*TW][SB], all
+// angles; lanes take consecutive (angle, column) pairs of the flattened
+// per-angle column ranges.
+template <int TW, int TH, int SB, int MODE, bool kExact>
+__global__ void __launch_bounds__(kTileBlock)
+k_bp_tile(TileArgs a) {
+  extern __shared__ float acc[];  // [TH*TW][SB]
+  const int N = a.g.N;
+  const int tiles_x = (N + TW - 1) / TW;
+  const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+  const int tx0 = tx * TW, ty0 = ty * TH;
+  const int tx1 = min(tx0 + TW, N), ty1 = min(ty0 + TH, N);
+  const int b = blockIdx.y;
+  const int nloc = TW * TH * SB;
+  for (int i = threadIdx.x; i < nloc; i += kTileBlock) acc[i] = 0.f;
+  __syncthreads();
+
+  const float x0 = grid_line(N, tx0), x1 = grid_line(N, tx1);
+  const float y0 = grid_line(N, ty0), y1 = grid_line(N, ty1);
+  const int C = a.g.C;
+  // lane t starts at flat pair t and advances by kTileBlock
+  int p = 0, k = threadIdx.x;
+  while (p < a.P && k >= a.count[p]) { k -= a.count[p]; ++p; }
+  while (p < a.P) {
+    int cmin, cmax;
+    const AngleInfo ang = a.ang[p];
+    tile_columns(ang, x0, x1, y0, y1, a.yi0, cmin, cmax);
+    const int c = cmin + k;
+    if (c >= 0 && c < C && c <= cmax) {
+      float wv[SB];
+      bool any = true;
+      if (MODE == kBpSino) {
+        any = false;
+#pragma unroll
+        for (int q = 0; q < SB; ++q) {
+          const int s = b * SB + q;
+          wv[q] = (s < a.S) ? a.w[((int64_t)p * a.S + s) * C + c] : 0.f;
+          any |= (wv[q] != 0.f);
+        }
+      }
+      if (any) {
+        const RayInfo r = a.rays[(int64_t)p * C + c];
+        traverse_window<kExact>(a.g, r, ang.quadrant, tx0, tx1, ty0, ty1,
+                                [&](int ix, int iy, float l, float) {
+                                  // ix == N is pixel 0 of the next row (truncation)
+                                  const bool wrap = (ix == N);
+                                  ix = wrap ? 0 : ix;
+                                  iy = wrap ? iy + 1 : iy;
+                                  const unsigned lx = (unsigned)(ix - tx0);
+                                  const unsigned ly = (unsigned)(iy - ty0);
+                                  if (lx < (unsigned)(tx1 - tx0) && ly < (unsigned)(ty1 - ty0)) {
+                                    float *dst = acc + (ly * TW + lx) * SB;
+                                    if (MODE == kBpSino) {
+#pragma unroll
+                                      for (int q = 0; q < SB; ++q) atomicAdd(dst + q, l * wv[q]);
+                                    } else if (MODE == kBpOnes) {
+                                      atomicAdd(dst, l);
+                                    } else {
+                                      atomicAdd(dst, 1.0f);
+                                    }
+                                  }
+                                });
+      }
+    }
+    k += kTileBlock;
+    while (p < a.P && k >= a.count[p]) { k -= a.count[p]; ++p; }
+  }
+  __syncthreads();
+  // coalesced write-out: plane by plane, row-contiguous
+  const int w_ = tx1 - tx0, h_ = ty1 - ty0;
+  for (int q = 0; q < SB; ++q) {
+    const int s = b * SB + q;
+    if (MODE == kBpSino && s >= a.S) break;
+    float *zs = a.z + (int64_t)s * N * N;
+    for (int i = threadIdx.x; i < w_ * h_; i += kTileBlock) {
+      const int ly = i / w_, lx = i - ly * w_;
+      zs[(int64_t)(ty0 + ly) * N + tx0 + lx] = acc[(ly * TW + lx) * SB + q];
+    }
+  }
+}
+
+// ------------------------------------------------------- pack / update --
+
+// xb[t/SB][j][t%SB] = src[(t + off)][j] for t < S_src_used, ghost value
+// outside: slots whose source slice index is < 0 or >= nsrc get `fill`.
+template <int SB>
+__global__ void __launch_bounds__(kPixBlock)
+k_pack(const float *__restrict__ src, int nsrc, int off, float fill, int64_t NN, int nslots,
+       float *__restrict__ xb) {
+  const int64_t j = (int64_t)blockIdx.x * kPixBlock + threadIdx.x;
+  const int b = blockIdx.y;
+  if (j >= NN) return;
+  float v[SB];
+#pragma unroll
+  for (int q = 0; q < SB; ++q) {
+    const int t = b * SB + q;
+    const int s = t + off;
+    v[q] = (t < nslots && s >= 0 && s < nsrc) ? src[(int64_t)s * NN + j] : fill;
+  }
+  float *dst = xb + ((int64_t)b * NN + j) * SB;
+#pragma unroll
+  for (int q = 0; q < SB; q += 4) *reinterpret_cast<float4 *>(dst + q) = make_float4(v[q], v[q + 1], v[q + 2], v[q + 3]);
+}
+
+// SIRT (sirt.cc:21-33) / MLEM (mlem.cc:21-34) update of a batch of slices
+// at pixel j, also refreshing the batched copy.
+template <int SB, bool kSIRT>
+__global__ void __launch_bounds__(kPixBlock)
+k_update_em(float *__restrict__ x, const float *__restrict__ z, const float *__restrict__ den,
+            int S, int64_t NN, float *__restrict__ xb) {
+  const int64_t j = (int64_t)blockIdx.x * kPixBlock + threadIdx.x;
+  const int b = blockIdx.y;
+  if (j >= NN) return;
+  const float d = den[j];
+  float v[SB];
+#pragma unroll
+  for (int q = 0; q < SB; ++q) {
+    const int s = b * SB + q;
+    v[q] = 0.f;
+    if (s < S) {
+      const int64_t o = (int64_t)s * NN + j;
+      float xv = x[o];
+      if (kSIRT) xv += z[o] / d;
+      else xv *= z[o] / d;
+      x[o] = xv;
+      v[q] = xv;
+    }
+  }
+  float *dst = xb + ((int64_t)b * NN + j) * SB;
+#pragma unroll
+  for (int q = 0; q < SB; q += 4) *reinterpret_cast<float4 *>(dst + q) = make_float4(v[q], v[q + 1], v[q + 2], v[q + 3]);
 }
 
 __global__ void k_fill(float *__restrict__ p, int64_t n, float v) {
   const int64_t i = (int64_t)blockIdx.x * kPixBlock + threadIdx.x;
   if (i < n) p[i] = v;
-}
-
-// sirt.cc:21-33
-__global__ void k_update_sirt(float *__restrict__ x, const float *__restrict__ num,
-                              const float *__restrict__ den, int64_t n, int64_t NN) {
-  const int64_t i = (int64_t)blockIdx.x * kPixBlock + threadIdx.x;
-  if (i < n) x[i] += num[i] / den[i % NN];
-}
-
-// mlem.cc:21-34
-__global__ void k_update_mlem(float *__restrict__ x, const float *__restrict__ num,
-                              const float *__restrict__ den, int64_t n, int64_t NN) {
-  const int64_t i = (int64_t)blockIdx.x * kPixBlock + threadIdx.x;
-  if (i < n) x[i] *= num[i] / den[i % NN];
 }
 
 // Pixel classes and neighbour orders of the regularisation stencils:
@@ -193,10 +327,11 @@ __device__ __forceinline__ int pixel_class(int m, int n, int N) {
 }
 
 // Fused PML stencil (pml.cc:25-201, F and G from zero) and closed-form
-// update (pml.cc:3-23):  x' = (-G + sqrt(G^2 - 8 num F)) / (4F).
+// update (pml.cc:3-23):  x' = (-G + sqrt(G^2 - 8 num F)) / (4F), with
+// num = -x * (A^T w) (the x factor of pml.cc:243 taken out of the sum).
 __global__ void __launch_bounds__(kPixBlock)
 k_update_pml(const float *__restrict__ x, float *__restrict__ xn,
-             const float *__restrict__ num, const float *__restrict__ den,
+             const float *__restrict__ z, const float *__restrict__ den,
              int S, int N, float beta, float *__restrict__ F_out,
              float *__restrict__ G_out) {
   const int m = blockIdx.x * kPixBlock + threadIdx.x;
@@ -217,7 +352,7 @@ k_update_pml(const float *__restrict__ x, float *__restrict__ xn,
   }
   const float G = gg + den[(int64_t)n * N + m];
   const float F = f;
-  const float nm = num[j];
+  const float nm = -(x0 * z[j]);
   if (F_out) { F_out[j] = F; G_out[j] = G; }
   xn[j] = (-G + __builtin_sqrtf(G * G - 8.0f * nm * F)) / (4.0f * F);
 }
@@ -225,14 +360,16 @@ k_update_pml(const float *__restrict__ x, float *__restrict__ xn,
 struct ApmlrArgs {
   StencilWeights w;
   RegParams p;
+  int compat;
 };
 
 // Fused APMLR stencil (apmlr.cc:919-982 with FGInner/FGTop/FGBottom) and
 // update (apmlr.cc:3-26) over ghost-padded images [S+2][N][N]; real slice s
-// lives at padded index s+1.
+// lives at padded index s+1.  The ray pass of slice s used padded slice s
+// (compat, apmlr.cc:1082-1084) or s+1, which also supplies num's x factor.
 __global__ void __launch_bounds__(kPixBlock)
 k_update_apmlr(const float *__restrict__ xp, float *__restrict__ xnp,
-               const float *__restrict__ num, const float *__restrict__ den,
+               const float *__restrict__ z, const float *__restrict__ den,
                int S, int N, ApmlrArgs a, float *__restrict__ F_out,
                float *__restrict__ G_out) {
   const int m = blockIdx.x * kPixBlock + threadIdx.x;
@@ -271,49 +408,76 @@ k_update_apmlr(const float *__restrict__ xp, float *__restrict__ xnp,
   }
   const float G = gg + den[(int64_t)n * N + m];
   const float F = f;
-  const float nm = num[j];
+  const float xpass = a.compat ? xp[jp - NN] : x0;
+  const float nm = -(xpass * z[j]);
   if (F_out) { F_out[j] = F; G_out[j] = G; }
   xnp[jp] = (-G + __builtin_sqrtf(G * G - 8.0f * nm * F)) / (4.0f * F);
 }
 
+// ------------------------------------------------- reference / debugging --
+
+// Whole-ray scatter with global atomics (z must be zeroed): the plain
+// ray-driven adjoint, kept as an independent check of the tile kernel.
+template <bool kExact>
+__global__ void __launch_bounds__(kRayBlock)
+k_adjoint_atomic(GridInfo g, const AngleInfo *__restrict__ ang,
+                 const RayInfo *__restrict__ rays, int P, int S, int mode,
+                 const float *__restrict__ r, float *__restrict__ z) {
+  const int64_t t = (int64_t)blockIdx.x * kRayBlock + threadIdx.x;
+  const int C = g.C;
+  if (t >= (int64_t)P * C) return;
+  const int c = (int)(t % C), p = (int)(t / C);
+  const int s = blockIdx.y;
+  const unsigned NN = (unsigned)g.N * (unsigned)g.N;
+  float w = 1.f;
+  if (mode == kBpSino) {
+    w = r[((int64_t)p * S + s) * C + c];
+    if (w == 0.f) return;
+  }
+  float *zs = z + (int64_t)s * NN;
+  traverse_full<kExact>(g, rays[t], ang[p].quadrant, [&](int ix, int iy, float l, float) {
+    const unsigned idx = (unsigned)(ix + iy * g.N);
+    if (idx < NN) atomic_add_f32(zs + idx, mode == kBpCount ? 1.0f : (mode == kBpOnes ? l : l * w));
+  });
+}
+
 template <bool kExact>
 __global__ void k_dump_rays(GridInfo g, const AngleInfo *__restrict__ ang,
-                            const float *__restrict__ yi, int n,
-                            const int *__restrict__ proj,
-                            const int *__restrict__ col, int max_seg,
-                            int *__restrict__ counts, int *__restrict__ idx,
+                            const RayInfo *__restrict__ rays, int n,
+                            const int *__restrict__ proj, const int *__restrict__ col,
+                            int max_seg, int *__restrict__ counts, int *__restrict__ idx,
                             float *__restrict__ len, float *__restrict__ len2) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   int k = 0;
   const int64_t base = (int64_t)t * max_seg;
-  int cnt = traverse_ray<kExact>(g, yi[col[t]], ang[proj[t]],
-                                 [&](int i, float l, float l2) {
-                                   if (k < max_seg) {
-                                     idx[base + k] = i;
-                                     len[base + k] = l;
-                                     len2[base + k] = l2;
-                                   }
-                                   ++k;
-                                 });
+  const int64_t ray = (int64_t)proj[t] * g.C + col[t];
+  const int cnt = traverse_full<kExact>(g, rays[ray], ang[proj[t]].quadrant,
+                                        [&](int ix, int iy, float l, float l2) {
+                                          if (k < max_seg) {
+                                            idx[base + k] = ix + iy * g.N;
+                                            len[base + k] = l;
+                                            len2[base + k] = l2;
+                                          }
+                                          ++k;
+                                        });
   counts[t] = cnt;
 }
 
 __global__ void __launch_bounds__(kRayBlock)
 k_count_segments(GridInfo g, const AngleInfo *__restrict__ ang,
-                 const float *__restrict__ yi, int P,
+                 const RayInfo *__restrict__ rays, int P,
                  unsigned long long *total, unsigned long long *oor) {
   const int64_t t = (int64_t)blockIdx.x * kRayBlock + threadIdx.x;
   const int C = g.C;
   unsigned cnt = 0, bad = 0;
   if (t < (int64_t)P * C) {
-    const int c = (int)(t % C), p = (int)(t / C);
+    const int p = (int)(t / C);
     const unsigned NN = (unsigned)g.N * (unsigned)g.N;
-    cnt = traverse_ray<true>(g, yi[c], ang[p], [&](int idx, float, float) {
-      bad += ((unsigned)idx >= NN) ? 1u : 0u;
+    cnt = traverse_full<true>(g, rays[t], ang[p].quadrant, [&](int ix, int iy, float, float) {
+      bad += ((unsigned)(ix + iy * g.N) >= NN) ? 1u : 0u;
     });
   }
-  // wavefront reduction, one atomic per wave
   for (int off = 32; off > 0; off >>= 1) {
     cnt += __shfl_down(cnt, off, 64);
     bad += __shfl_down(bad, off, 64);
@@ -335,130 +499,204 @@ __global__ void k_check_finite(const float *__restrict__ v, int64_t n,
 
 inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// ------------------------------------------------ launch-shape dispatch --
+
+template <int SB, bool E>
+hipError_t fwd_mode(int mode, const Geometry &g, const RayInfo *rays, const float *xb,
+                    const float *sino, float *out, hipStream_t st) {
+  const int64_t n = g.rays_per_slice();
+  dim3 grid(blocks_for(n, kRayBlock), (g.S + SB - 1) / SB), block(kRayBlock);
+  if (mode == kFwdY)
+    hipLaunchKernelGGL((k_forward<SB, kFwdY, E>), grid, block, 0, st, g.grid, g.d_ang, rays,
+                       g.P, g.S, xb, sino, out);
+  else if (mode == kFwdSIRT)
+    hipLaunchKernelGGL((k_forward<SB, kFwdSIRT, E>), grid, block, 0, st, g.grid, g.d_ang,
+                       rays, g.P, g.S, xb, sino, out);
+  else
+    hipLaunchKernelGGL((k_forward<SB, kFwdEM, E>), grid, block, 0, st, g.grid, g.d_ang, rays,
+                       g.P, g.S, xb, sino, out);
+  return hipGetLastError();
+}
+
+template <int TW, int TH, int SB, int MODE, bool E>
+hipError_t bp_launch(const TileArgs &a, int nbatch, hipStream_t st) {
+  const int N = a.g.N;
+  const int tiles = ((N + TW - 1) / TW) * ((N + TH - 1) / TH);
+  const size_t lds = (size_t)TW * TH * SB * sizeof(float);
+  auto kern = k_bp_tile<TW, TH, SB, MODE, E>;
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(kern, dim3(tiles, nbatch), dim3(kTileBlock), lds, st, a);
+  return hipGetLastError();
+}
+
+template <int TW, int TH, int SB>
+hipError_t bp_mode(int mode, bool exact, const TileArgs &a, int nbatch, hipStream_t st) {
+  if (mode == kBpSino)
+    return exact ? bp_launch<TW, TH, SB, kBpSino, true>(a, nbatch, st)
+                 : bp_launch<TW, TH, SB, kBpSino, false>(a, nbatch, st);
+  if (mode == kBpOnes)
+    return exact ? bp_launch<TW, TH, 1, kBpOnes, true>(a, 1, st)
+                 : bp_launch<TW, TH, 1, kBpOnes, false>(a, 1, st);
+  return bp_launch<TW, TH, 1, kBpCount, true>(a, 1, st);
+}
+
 }  // namespace
 
-hipError_t launch_den(const Geometry &g, float *den, hipStream_t st, bool exact) {
+// ------------------------------------------------------------ launchers --
+
+int tile_count_bound(float sinq, float cosq, int TW, int TH) {
+  // columns in an interval of length TW|sin| + TH|cos| + 2 (margins): at
+  // most floor(len) + 1; one more for rounding of the float projections
+  const double len = TW * std::fabs((double)sinq) + TH * std::fabs((double)cosq) + 2.0;
+  return (int)std::floor(len) + 2;
+}
+
+hipError_t launch_ray_info(const Geometry &g, RayInfo *rays, hipStream_t st) {
   const int64_t n = g.rays_per_slice();
-  if (exact)
-    hipLaunchKernelGGL(k_den<true>, dim3(blocks_for(n, kRayBlock)), dim3(kRayBlock), 0,
-                       st, g.grid, g.d_ang, g.d_yi, g.P, den);
-  else
-    hipLaunchKernelGGL(k_den<false>, dim3(blocks_for(n, kRayBlock)), dim3(kRayBlock), 0,
-                       st, g.grid, g.d_ang, g.d_yi, g.P, den);
+  hipLaunchKernelGGL(k_ray_info, dim3(blocks_for(n, kRayBlock)), dim3(kRayBlock), 0, st,
+                     g.grid, g.d_ang, g.d_yi, g.P, rays);
   return hipGetLastError();
 }
 
-hipError_t launch_forward(const Geometry &g, const float *x, int64_t x_stride,
-                          float *y, hipStream_t st, bool exact) {
-  const int64_t n = (int64_t)g.P * g.S * g.C;
-  if (exact)
-    hipLaunchKernelGGL(k_forward<true>, dim3(blocks_for(n, kRayBlock)), dim3(kRayBlock),
-                       0, st, g.grid, g.d_ang, g.d_yi, g.P, g.S, x, x_stride, y);
-  else
-    hipLaunchKernelGGL(k_forward<false>, dim3(blocks_for(n, kRayBlock)), dim3(kRayBlock),
-                       0, st, g.grid, g.d_ang, g.d_yi, g.P, g.S, x, x_stride, y);
-  return hipGetLastError();
-}
-
-hipError_t launch_adjoint(const Geometry &g, const float *r, float *z,
-                          hipStream_t st, bool exact) {
-  const int64_t n = (int64_t)g.P * g.S * g.C;
-  if (exact)
-    hipLaunchKernelGGL(k_adjoint<true>, dim3(blocks_for(n, kRayBlock)), dim3(kRayBlock),
-                       0, st, g.grid, g.d_ang, g.d_yi, g.P, g.S, r, z);
-  else
-    hipLaunchKernelGGL(k_adjoint<false>, dim3(blocks_for(n, kRayBlock)), dim3(kRayBlock),
-                       0, st, g.grid, g.d_ang, g.d_yi, g.P, g.S, r, z);
-  return hipGetLastError();
-}
-
-template <int ALG>
-static void ray_pass_alg(const Geometry &g, const float *x, int64_t x_stride,
-                         const float *sino, float *num, hipStream_t st, bool exact) {
-  const int64_t n = (int64_t)g.P * g.S * g.C;
-  dim3 grid(blocks_for(n, kRayBlock)), block(kRayBlock);
-  if (exact)
-    hipLaunchKernelGGL((k_ray_pass<ALG, true>), grid, block, 0, st, g.grid, g.d_ang,
-                       g.d_yi, g.P, g.S, x, x_stride, sino, num);
-  else
-    hipLaunchKernelGGL((k_ray_pass<ALG, false>), grid, block, 0, st, g.grid, g.d_ang,
-                       g.d_yi, g.P, g.S, x, x_stride, sino, num);
-}
-
-hipError_t launch_ray_pass(int alg, const Geometry &g, const float *x,
-                           int64_t x_stride, const float *sino, float *num,
-                           hipStream_t st, bool exact) {
-  switch (alg) {
-    case kSIRT: ray_pass_alg<kSIRT>(g, x, x_stride, sino, num, st, exact); break;
-    case kMLEM: ray_pass_alg<kMLEM>(g, x, x_stride, sino, num, st, exact); break;
-    default: ray_pass_alg<kPML>(g, x, x_stride, sino, num, st, exact); break;
+hipError_t launch_forward(const Geometry &g, const RayInfo *rays, int sb, int mode,
+                          bool exact, const float *xb, const float *sino, float *out,
+                          hipStream_t st) {
+  switch (sb) {
+    case 4: return exact ? fwd_mode<4, true>(mode, g, rays, xb, sino, out, st)
+                         : fwd_mode<4, false>(mode, g, rays, xb, sino, out, st);
+    case 8: return exact ? fwd_mode<8, true>(mode, g, rays, xb, sino, out, st)
+                         : fwd_mode<8, false>(mode, g, rays, xb, sino, out, st);
+    case 16: return exact ? fwd_mode<16, true>(mode, g, rays, xb, sino, out, st)
+                          : fwd_mode<16, false>(mode, g, rays, xb, sino, out, st);
+    default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_backproject(const Geometry &g, const RayInfo *rays, const int *count,
+                              const TileShape &ts, int mode, bool exact, const float *w,
+                              float *z, hipStream_t st) {
+  TileArgs a;
+  a.g = g.grid;
+  a.ang = g.d_ang;
+  a.rays = rays;
+  a.count = count;
+  a.P = g.P;
+  a.S = g.S;
+  a.yi0 = g.yi0;
+  a.w = w;
+  a.z = z;
+  const int nb = (g.S + ts.sb - 1) / ts.sb;
+#define TRACE_BP_CASE(TW_, TH_, SB_)                              \
+  if (ts.tw == TW_ && ts.th == TH_ && ts.sb == SB_)               \
+    return bp_mode<TW_, TH_, SB_>(mode, exact, a, nb, st);
+  TRACE_BP_CASE(64, 64, 4)
+  TRACE_BP_CASE(32, 32, 4)
+  TRACE_BP_CASE(64, 32, 4)
+  TRACE_BP_CASE(32, 64, 4)
+  TRACE_BP_CASE(32, 32, 8)
+  TRACE_BP_CASE(64, 32, 8)
+  TRACE_BP_CASE(64, 64, 8)
+  TRACE_BP_CASE(32, 32, 16)
+#undef TRACE_BP_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_pack(const float *src, int nsrc, int off, float fill, int64_t NN, int nslots,
+                       int sb, float *xb, hipStream_t st) {
+  const int nb = (nslots + sb - 1) / sb;
+  dim3 grid(blocks_for(NN, kPixBlock), nb), block(kPixBlock);
+  switch (sb) {
+    case 4: hipLaunchKernelGGL(k_pack<4>, grid, block, 0, st, src, nsrc, off, fill, NN, nslots, xb); break;
+    case 8: hipLaunchKernelGGL(k_pack<8>, grid, block, 0, st, src, nsrc, off, fill, NN, nslots, xb); break;
+    case 16: hipLaunchKernelGGL(k_pack<16>, grid, block, 0, st, src, nsrc, off, fill, NN, nslots, xb); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_update_em(bool sirt, float *x, const float *z, const float *den, int S,
+                            int64_t NN, int sb, float *xb, hipStream_t st) {
+  const int nb = (S + sb - 1) / sb;
+  dim3 grid(blocks_for(NN, kPixBlock), nb), block(kPixBlock);
+#define TRACE_UPD_CASE(SB_)                                                                   \
+  case SB_:                                                                                   \
+    if (sirt)                                                                                 \
+      hipLaunchKernelGGL((k_update_em<SB_, true>), grid, block, 0, st, x, z, den, S, NN, xb); \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_update_em<SB_, false>), grid, block, 0, st, x, z, den, S, NN, xb); \
+    break;
+  switch (sb) {
+    TRACE_UPD_CASE(4)
+    TRACE_UPD_CASE(8)
+    TRACE_UPD_CASE(16)
+    default: return hipErrorInvalidValue;
+  }
+#undef TRACE_UPD_CASE
   return hipGetLastError();
 }
 
 hipError_t launch_fill(float *p, int64_t n, float v, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fill, dim3(blocks_for(n, kPixBlock)), dim3(kPixBlock), 0, st, p,
-                     n, v);
+  hipLaunchKernelGGL(k_fill, dim3(blocks_for(n, kPixBlock)), dim3(kPixBlock), 0, st, p, n, v);
   return hipGetLastError();
 }
 
-hipError_t launch_update_sirt(float *x, const float *num, const float *den, int S,
-                              int64_t NN, hipStream_t st) {
-  const int64_t n = (int64_t)S * NN;
-  hipLaunchKernelGGL(k_update_sirt, dim3(blocks_for(n, kPixBlock)), dim3(kPixBlock), 0,
-                     st, x, num, den, n, NN);
-  return hipGetLastError();
-}
-
-hipError_t launch_update_mlem(float *x, const float *num, const float *den, int S,
-                              int64_t NN, hipStream_t st) {
-  const int64_t n = (int64_t)S * NN;
-  hipLaunchKernelGGL(k_update_mlem, dim3(blocks_for(n, kPixBlock)), dim3(kPixBlock), 0,
-                     st, x, num, den, n, NN);
-  return hipGetLastError();
-}
-
-hipError_t launch_update_pml(const float *x, float *xn, const float *num,
-                             const float *den, int S, int N, float beta,
-                             hipStream_t st, float *F_out, float *G_out) {
+hipError_t launch_update_pml(const float *x, float *xn, const float *z, const float *den,
+                             int S, int N, float beta, hipStream_t st, float *F_out,
+                             float *G_out) {
   dim3 grid(blocks_for(N, kPixBlock), N, S);
-  hipLaunchKernelGGL(k_update_pml, grid, dim3(kPixBlock), 0, st, x, xn, num, den, S,
-                     N, beta, F_out, G_out);
+  hipLaunchKernelGGL(k_update_pml, grid, dim3(kPixBlock), 0, st, x, xn, z, den, S, N, beta,
+                     F_out, G_out);
   return hipGetLastError();
 }
 
-hipError_t launch_update_apmlr(const float *xpad, float *xnpad, const float *num,
+hipError_t launch_update_apmlr(const float *xpad, float *xnpad, const float *z,
                                const float *den, int S, int N, const RegParams &p,
-                               const StencilWeights &w, hipStream_t st,
+                               const StencilWeights &w, int compat, hipStream_t st,
                                float *F_out, float *G_out) {
   dim3 grid(blocks_for(N, kPixBlock), N, S);
-  ApmlrArgs a{w, p};
-  hipLaunchKernelGGL(k_update_apmlr, grid, dim3(kPixBlock), 0, st, xpad, xnpad, num,
-                     den, S, N, a, F_out, G_out);
+  ApmlrArgs a{w, p, compat};
+  hipLaunchKernelGGL(k_update_apmlr, grid, dim3(kPixBlock), 0, st, xpad, xnpad, z, den, S,
+                     N, a, F_out, G_out);
   return hipGetLastError();
 }
 
-hipError_t launch_dump_rays(const Geometry &g, int n, const int *proj, const int *col,
-                            int max_seg, int *counts, int *idx, float *len,
+hipError_t launch_adjoint_atomic(const Geometry &g, const RayInfo *rays, int mode, bool exact,
+                                 const float *r, float *z, int nslices, hipStream_t st) {
+  const int64_t n = g.rays_per_slice();
+  dim3 grid(blocks_for(n, kRayBlock), nslices), block(kRayBlock);
+  if (exact)
+    hipLaunchKernelGGL(k_adjoint_atomic<true>, grid, block, 0, st, g.grid, g.d_ang, rays, g.P,
+                       g.S, mode, r, z);
+  else
+    hipLaunchKernelGGL(k_adjoint_atomic<false>, grid, block, 0, st, g.grid, g.d_ang, rays, g.P,
+                       g.S, mode, r, z);
+  return hipGetLastError();
+}
+
+hipError_t launch_dump_rays(const Geometry &g, const RayInfo *rays, int n, const int *proj,
+                            const int *col, int max_seg, int *counts, int *idx, float *len,
                             float *len2, hipStream_t st, bool exact) {
   if (n <= 0) return hipSuccess;
   if (exact)
-    hipLaunchKernelGGL(k_dump_rays<true>, dim3(blocks_for(n, 64)), dim3(64), 0, st,
-                       g.grid, g.d_ang, g.d_yi, n, proj, col, max_seg, counts, idx,
-                       len, len2);
+    hipLaunchKernelGGL(k_dump_rays<true>, dim3(blocks_for(n, 64)), dim3(64), 0, st, g.grid,
+                       g.d_ang, rays, n, proj, col, max_seg, counts, idx, len, len2);
   else
-    hipLaunchKernelGGL(k_dump_rays<false>, dim3(blocks_for(n, 64)), dim3(64), 0, st,
-                       g.grid, g.d_ang, g.d_yi, n, proj, col, max_seg, counts, idx,
-                       len, len2);
+    hipLaunchKernelGGL(k_dump_rays<false>, dim3(blocks_for(n, 64)), dim3(64), 0, st, g.grid,
+                       g.d_ang, rays, n, proj, col, max_seg, counts, idx, len, len2);
   return hipGetLastError();
 }
 
-hipError_t launch_count_segments(const Geometry &g, unsigned long long *total,
-                                 unsigned long long *oor, hipStream_t st) {
+hipError_t launch_count_segments(const Geometry &g, const RayInfo *rays,
+                                 unsigned long long *total, unsigned long long *oor,
+                                 hipStream_t st) {
   const int64_t n = g.rays_per_slice();
-  hipLaunchKernelGGL(k_count_segments, dim3(blocks_for(n, kRayBlock)), dim3(kRayBlock),
-                     0, st, g.grid, g.d_ang, g.d_yi, g.P, total, oor);
+  hipLaunchKernelGGL(k_count_segments, dim3(blocks_for(n, kRayBlock)), dim3(kRayBlock), 0, st,
+                     g.grid, g.d_ang, rays, g.P, total, oor);
   return hipGetLastError();
 }
 

@@ -1,26 +1,32 @@
 // traverse.h -- on-the-fly ray traversal for gfx950, bit-compatible with the
 // reference's per-ray pipeline
-//   CalculateCoordinates  trace_utils.cc:48-80
-//   MergeTrimCoordinates  trace_utils.cc:82-111
-//   SortIntersectionPoints trace_utils.cc:113-158
+//   CalculateCoordinates     trace_utils.cc:48-80
+//   MergeTrimCoordinates     trace_utils.cc:82-111
+//   SortIntersectionPoints   trace_utils.cc:113-158
 //   CalculateDistanceLengths trace_utils.cc:160-187
 // without materialising any per-ray array.
 //
-// The reference evaluates all 2(N+1) grid-line crossings, keeps those strictly
-// inside the grid square, and merges the two lists by x.  Here:
+// The reference evaluates all 2(N+1) grid-line crossings of a ray, keeps
+// those strictly inside the grid square, and merges the two lists by x.
+// Here:
 //
 // * the crossing with grid line n is recomputed from the same global formula
-//   (islope*(gy[n]-srcy)+srcx, slope*(gx[n]-srcx)+srcy) whenever it is
-//   needed, with FMA contraction off, so every coordinate is bit-identical;
+//   (islope*(gy[n]-srcy)+srcx for the A list, slope*(gx[n]-srcx)+srcy for
+//   the B list) whenever it is needed, with FMA contraction off, so every
+//   coordinate is bit-identical to the reference's;
 // * each crossing sequence is monotone in n (every step is a correctly
 //   rounded, monotone function of the previous value), so the trimmed lists
-//   are contiguous index ranges, found by binary search in O(log N) instead
-//   of the reference's O(N) scan;
-// * the merge runs as a two-pointer walk that emits one segment per step.
+//   are contiguous index ranges, found once per ray by binary search
+//   (RayInfo) instead of the reference's O(N) scan;
+// * the merge is a two-pointer walk that emits one segment per step and can
+//   be resumed at any element: when the A list is ascending in walk order
+//   (always, for angles in [0, pi)), the merged list is sorted by x with ties
+//   going to B, so the state just after any element is known in closed form.
+//   This lets an image-tile kernel walk only the part of a ray near its tile.
 //
-// The pixel index uses the reference's truncating conversion, so an index
-// can fall on N (a midpoint rounding onto the far grid line); callers guard
-// the image bounds.
+// The pixel index uses the reference's truncating conversion, so a midpoint
+// that rounds onto the far grid line gives ix == N, i.e. pixel 0 of the next
+// row, as in the reference; callers guard the image bounds.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -39,11 +45,19 @@ struct AngleInfo {
 };
 
 struct GridInfo {
-  int N;         // grid side (= C)
-  int C;         // detector columns
-  float gx0;     // gridx[0]
-  float gxN;     // gridx[N]
-  float mg;      // (float)(N / 2.0), trace_utils.cc:220
+  int N;      // grid side (= C)
+  int C;      // detector columns
+  float gx0;  // gridx[0]
+  float gxN;  // gridx[N]
+  float mg;   // (float)(N / 2.0), trace_utils.cc:220
+};
+
+// Per-ray constants: source point, slopes and the trimmed list ranges
+// (A: horizontal-line crossings n in [alo, alo+alen); B: vertical-line
+// crossings n in [blo, blo+blen)).  Computed once per geometry.
+struct RayInfo {
+  float srcx, srcy, slope, islope;
+  int alo, alen, blo, blen;
 };
 
 // gridx[n] = gridy[n] = (float)(-N/2.0 + n) (trace_data.h:103-108), written
@@ -61,67 +75,31 @@ __device__ __forceinline__ float seg_sqrt(float v) {
   }
 }
 
-// Ray set-up (trace_utils.cc:63-75 with xi = -1e6, yi from sirt.cc:129).
-struct RaySetup {
-  float srcx, srcy, slope, islope;
-};
-
-__device__ __forceinline__ RaySetup ray_setup(float yi, float sinq, float cosq) {
-  const float xi = -1e6f;
-  RaySetup r;
-  float xc = xi * cosq;
-  float ys = yi * sinq;
-  float xs = xi * sinq;
-  float yc = yi * cosq;
-  r.srcx = xc - ys;
-  r.srcy = xs + yc;
-  float detx = -(xc + ys);
-  float dety = -xs + yc;
-  r.slope = (r.srcy - dety) / (r.srcx - detx);
-  r.islope = 1.0f / r.slope;
-  return r;
-}
-
-// Crossing of the ray with horizontal line y = gy[n] (the A list) and with
-// vertical line x = gx[n] (the B list).
-__device__ __forceinline__ float cross_a(const RaySetup &r, int N, int n) {
-  return r.islope * (grid_line(N, n) - r.srcy) + r.srcx;
-}
-__device__ __forceinline__ float cross_b(const RaySetup &r, int N, int n) {
-  return r.slope * (grid_line(N, n) - r.srcx) + r.srcy;
-}
-
-// Contiguous range [lo, hi] of n in [0, N] whose crossing lies strictly
-// inside (gx0, gxN).  `k` is the multiplier of the sequence (islope or
-// slope), `b0`/`b1` the subtrahend and addend (srcy, srcx for A).  Returns
-// the range length (0 if empty).
+// Contiguous range of n in [0, N] whose crossing k*(gl(n)-b0)+b1 lies
+// strictly inside (gx0, gxN) (trace_utils.cc:98-109).  Returns the length.
 __device__ __forceinline__ int trim_range(float k, float b0, float b1, int N,
                                           float gx0, float gxN, int &lo) {
   // A non-finite multiplier or offset makes every crossing inf or NaN, which
-  // the strict trims of trace_utils.cc:98-109 all reject.
+  // the strict trims all reject.
   if (!__builtin_isfinite(k) || !__builtin_isfinite(b0) || !__builtin_isfinite(b1)) {
     lo = 0;
     return 0;
   }
   auto f = [&](int n) { return k * (grid_line(N, n) - b0) + b1; };
   const bool inc = (k >= 0.0f);  // +-0 gives a constant sequence
-  // first/last index satisfying f > gx0 and f < gxN (monotone predicates)
   int a, b;
   if (inc) {
-    // lower end: first n with f(n) > gx0
-    int L = 0, H = N + 1;
+    int L = 0, H = N + 1;  // first n with f(n) > gx0
     while (L < H) { int m = (L + H) >> 1; if (f(m) > gx0) H = m; else L = m + 1; }
     a = L;
-    // upper end: last n with f(n) < gxN
-    L = -1; H = N;
+    L = -1; H = N;  // last n with f(n) < gxN
     while (L < H) { int m = (L + H + 1) >> 1; if (f(m) < gxN) L = m; else H = m - 1; }
     b = L;
   } else {
-    // decreasing: f > gx0 holds on a prefix, f < gxN on a suffix
-    int L = -1, H = N;
+    int L = -1, H = N;  // last n with f(n) > gx0
     while (L < H) { int m = (L + H + 1) >> 1; if (f(m) > gx0) L = m; else H = m - 1; }
     b = L;
-    L = 0; H = N + 1;
+    L = 0; H = N + 1;  // first n with f(n) < gxN
     while (L < H) { int m = (L + H) >> 1; if (f(m) < gxN) H = m; else L = m + 1; }
     a = L;
   }
@@ -129,65 +107,222 @@ __device__ __forceinline__ int trim_range(float k, float b0, float b1, int N,
   return (b >= a) ? (b - a + 1) : 0;
 }
 
-// Walks one ray and calls visit(idx, len, len2) for each of its len-1
-// segments in the reference's order.  Returns the segment count.
-template <bool kExactSqrt, typename Visit>
-__device__ __forceinline__ int traverse_ray(const GridInfo &g, float yi,
-                                            const AngleInfo &ang, Visit &&visit) {
-  const int N = g.N;
-  RaySetup r = ray_setup(yi, ang.sinq, ang.cosq);
-  int alo, blo;
-  const int alen = trim_range(r.islope, r.srcy, r.srcx, N, g.gx0, g.gxN, alo);
-  const int blen = trim_range(r.slope, r.srcx, r.srcy, N, g.gx0, g.gxN, blo);
-  // A is walked forward when quadrant==1, backward otherwise
-  // (trace_utils.cc:175,183,197).
-  const int dA = ang.quadrant ? 1 : -1;
-  int nA = ang.quadrant ? alo : (alo + alen - 1);
-  int nB = blo;
-  int i = 0, j = 0;
-  // Head of each list.  Heads past the end of a list are computed but never
-  // selected.
-  float ax = cross_a(r, N, nA), ay = grid_line(N, nA);
-  float bx = grid_line(N, nB), by = cross_b(r, N, nB);
-  float px = 0.f, py = 0.f;
-  const int total = alen + blen;
-  int nseg = 0;
-  for (int k = 0; k < total; ++k) {
-    // merge step: A wins only on a strict ax < bx (trace_utils.cc:184)
+// Ray set-up (trace_utils.cc:63-75 with xi = -1e6, yi from sirt.cc:129) and
+// trims.
+__device__ __forceinline__ RayInfo make_ray(const GridInfo &g, float yi, float sinq,
+                                            float cosq) {
+  const float xi = -1e6f;
+  RayInfo r;
+  const float xc = xi * cosq;
+  const float ys = yi * sinq;
+  const float xs = xi * sinq;
+  const float yc = yi * cosq;
+  r.srcx = xc - ys;
+  r.srcy = xs + yc;
+  const float detx = -(xc + ys);
+  const float dety = -xs + yc;
+  r.slope = (r.srcy - dety) / (r.srcx - detx);
+  r.islope = 1.0f / r.slope;
+  r.alen = trim_range(r.islope, r.srcy, r.srcx, g.N, g.gx0, g.gxN, r.alo);
+  r.blen = trim_range(r.slope, r.srcx, r.srcy, g.N, g.gx0, g.gxN, r.blo);
+  return r;
+}
+
+// Two-pointer merge of the A and B lists of one ray (trace_utils.cc:113-158),
+// resumable at any element.
+struct Walker {
+  float srcx, srcy, slope, islope;
+  int N, alo, alen, blo, blen, dA;
+  bool fwd;
+  int i, j, nA, nB;     // A/B elements emitted so far; grid lines of the heads
+  float ax, ay, bx, by; // heads (computed even past the end, never selected)
+
+  __device__ __forceinline__ void init(const RayInfo &r, int N_, int quadrant) {
+    srcx = r.srcx; srcy = r.srcy; slope = r.slope; islope = r.islope;
+    N = N_; alo = r.alo; alen = r.alen; blo = r.blo; blen = r.blen;
+    fwd = quadrant != 0;  // A walked forward iff quadrant == 1 (:175,183,197)
+    dA = fwd ? 1 : -1;
+  }
+  __device__ __forceinline__ int a_line(int wi) const {
+    return fwd ? alo + wi : alo + alen - 1 - wi;
+  }
+  __device__ __forceinline__ float cross_a(int n) const {
+    return islope * (grid_line(N, n) - srcy) + srcx;
+  }
+  __device__ __forceinline__ float cross_b(int n) const {
+    return slope * (grid_line(N, n) - srcx) + srcy;
+  }
+  // A list ascending in walk order (then the merged list is sorted by x).
+  __device__ __forceinline__ bool a_sorted() const {
+    return alen < 2 || cross_a(a_line(0)) <= cross_a(a_line(alen - 1));
+  }
+  // Positions the walker after i_ A and j_ B elements.
+  __device__ __forceinline__ void seek(int i_, int j_) {
+    i = i_; j = j_;
+    nA = a_line(i); nB = blo + j;
+    ax = cross_a(nA); ay = grid_line(N, nA);
+    bx = grid_line(N, nB); by = cross_b(nB);
+  }
+  // Number of A elements (walk order, assumed ascending) with x < v.
+  __device__ __forceinline__ int count_a_lt(float v) const {
+    int L = 0, H = alen;
+    while (L < H) { int m = (L + H) >> 1; if (cross_a(a_line(m)) < v) L = m + 1; else H = m; }
+    return L;
+  }
+  // Number of B elements with x <= v.
+  __device__ __forceinline__ int count_b_le(float v) const {
+    int L = 0, H = blen;
+    while (L < H) { int m = (L + H) >> 1; if (grid_line(N, blo + m) <= v) L = m + 1; else H = m; }
+    return L;
+  }
+  // Emits the next merged element: A wins only on a strict ax < bx
+  // (trace_utils.cc:184).  Branch-free across the wavefront.
+  __device__ __forceinline__ void next(float &qx, float &qy) {
     const bool takeA = (i < alen) && (j >= blen || ax < bx);
-    const float qx = takeA ? ax : bx;
-    const float qy = takeA ? ay : by;
-    // advance the list that was consumed and recompute its head with the
-    // same formula for either list (branch-free across the wavefront)
+    qx = takeA ? ax : bx;
+    qy = takeA ? ay : by;
     i += takeA ? 1 : 0;
     j += takeA ? 0 : 1;
     nA += takeA ? dA : 0;
     nB += takeA ? 0 : 1;
     const int nn = takeA ? nA : nB;
-    const float kk = takeA ? r.islope : r.slope;
-    const float b0 = takeA ? r.srcy : r.srcx;
-    const float b1 = takeA ? r.srcx : r.srcy;
+    const float kk = takeA ? islope : slope;
+    const float b0 = takeA ? srcy : srcx;
+    const float b1 = takeA ? srcx : srcy;
     const float gl = grid_line(N, nn);
     const float cr = kk * (gl - b0) + b1;
     ax = takeA ? cr : ax;
     ay = takeA ? gl : ay;
     bx = takeA ? bx : gl;
     by = takeA ? by : cr;
-    if (k > 0) {
-      const float dx = qx - px;
-      const float dy = qy - py;
-      const float mx = (qx + px) * 0.5f;
-      const float my = (qy + py) * 0.5f;
-      const float l2 = dx * dx + dy * dy;
-      const float l = seg_sqrt<kExactSqrt>(l2);
-      const int ix = (int)(mx + g.mg);
-      const int iy = (int)(my + g.mg);
-      visit(ix + iy * N, l, l2);
+  }
+};
+
+// Segment between consecutive merged points (trace_utils.cc:160-187).
+template <bool kExactSqrt>
+__device__ __forceinline__ void segment(float px, float py, float qx, float qy, float mg,
+                                        int &ix, int &iy, float &len, float &len2) {
+  const float dx = qx - px;
+  const float dy = qy - py;
+  const float mx = (qx + px) * 0.5f;
+  const float my = (qy + py) * 0.5f;
+  len2 = dx * dx + dy * dy;
+  len = seg_sqrt<kExactSqrt>(len2);
+  ix = (int)(mx + mg);
+  iy = (int)(my + mg);
+}
+
+// Runs the merge from the walker's state, with an optional previous point,
+// until (i >= i_stop && j >= j_stop); visit(ix, iy, len, len2) per segment.
+template <bool kExactSqrt, typename Visit>
+__device__ __forceinline__ int walk(Walker &w, float mg, bool has_prev, float px, float py,
+                                    int i_stop, int j_stop, Visit &&visit) {
+  int nseg = 0;
+  const int limit = w.alen + w.blen;
+  for (int k = 0; k < limit; ++k) {
+    if (w.i >= i_stop && w.j >= j_stop) break;
+    float qx, qy;
+    w.next(qx, qy);
+    if (has_prev) {
+      int ix, iy;
+      float l, l2;
+      segment<kExactSqrt>(px, py, qx, qy, mg, ix, iy, l, l2);
+      visit(ix, iy, l, l2);
       ++nseg;
     }
-    px = qx; py = qy;
+    has_prev = true;
+    px = qx;
+    py = qy;
   }
   return nseg;
+}
+
+// Whole ray in the reference's order.
+template <bool kExactSqrt, typename Visit>
+__device__ __forceinline__ int traverse_full(const GridInfo &g, const RayInfo &r, int quadrant,
+                                             Visit &&visit) {
+  Walker w;
+  w.init(r, g.N, quadrant);
+  w.seek(0, 0);
+  return walk<kExactSqrt>(w, g.mg, false, 0.f, 0.f, w.alen, w.blen, visit);
+}
+
+// Rays whose merged points are used for a window of grid lines: every
+// segment whose midpoint falls in the lines' span [lo, hi] is produced,
+// together with a few neighbours (the callers filter by pixel).
+constexpr int kWindowMargin = 3;
+
+// Segments of the ray that can have their midpoint in pixel columns
+// [tx0, tx1) x rows [ty0, ty1): the ray is resumed just after the last A (or
+// B) element that lies kWindowMargin lines before the window and stopped
+// kWindowMargin lines after it.  Rays with |islope| <= 1 cross more
+// horizontal lines than vertical ones and are windowed on the A list (rows),
+// others on the B list (columns).  Consecutive merged points lie within
+// about one line of each other along the window axis (the crossing noise of
+// the reference's far source point is below 0.2 pixel), so the margin
+// covers every segment of the tile.  A rays whose A list is not ascending
+// (angles outside [0, pi) where the quadrant rule and the slope sign
+// disagree) are walked whole.
+template <bool kExactSqrt, typename Visit>
+__device__ __forceinline__ int traverse_window(const GridInfo &g, const RayInfo &r, int quadrant,
+                                               int tx0, int tx1, int ty0, int ty1,
+                                               Visit &&visit) {
+  Walker w;
+  w.init(r, g.N, quadrant);
+  if (!w.a_sorted()) {
+    w.seek(0, 0);
+    return walk<kExactSqrt>(w, g.mg, false, 0.f, 0.f, w.alen, w.blen, visit);
+  }
+  const int M = kWindowMargin;
+  if (__builtin_fabsf(r.islope) <= 1.0f) {
+    // window on A (rows): lines [ty0 - M, ty1 + M] as walk indices
+    int iS, iE;
+    if (w.fwd) {
+      iS = (ty0 - M) - w.alo;
+      iE = (ty1 + M) - w.alo + 1;
+    } else {
+      iS = (w.alo + w.alen - 1) - (ty1 + M);
+      iE = (w.alo + w.alen - 1) - (ty0 - M) + 1;
+    }
+    iS = max(0, min(iS, w.alen));
+    iE = max(0, min(iE, w.alen));
+    if (iS >= iE) return 0;
+    bool has_prev = false;
+    float px = 0.f, py = 0.f;
+    int j0 = 0;
+    if (iS > 0) {
+      const int n = w.a_line(iS - 1);
+      px = w.cross_a(n);
+      py = grid_line(g.N, n);
+      j0 = w.count_b_le(px);  // B elements merged before A[iS-1] (ties: B first)
+      has_prev = true;
+    }
+    w.seek(iS, j0);
+    const int i_stop = iE;
+    const int j_stop = (iE == w.alen) ? w.blen : 0;
+    return walk<kExactSqrt>(w, g.mg, has_prev, px, py, i_stop, j_stop, visit);
+  } else {
+    // window on B (columns): lines [tx0 - M, tx1 + M]
+    int jS = (tx0 - M) - w.blo;
+    int jE = (tx1 + M) - w.blo + 1;
+    jS = max(0, min(jS, w.blen));
+    jE = max(0, min(jE, w.blen));
+    if (jS >= jE) return 0;
+    bool has_prev = false;
+    float px = 0.f, py = 0.f;
+    int i0 = 0;
+    if (jS > 0) {
+      const int n = w.blo + jS - 1;
+      px = grid_line(g.N, n);
+      py = w.cross_b(n);
+      i0 = w.count_a_lt(px);  // A elements merged before B[jS-1] (strict <)
+      has_prev = true;
+    }
+    w.seek(i0, jS);
+    const int i_stop = (jE == w.blen) ? w.alen : 0;
+    const int j_stop = jE;
+    return walk<kExactSqrt>(w, g.mg, has_prev, px, py, i_stop, j_stop, visit);
+  }
 }
 
 }  // namespace trace_hip

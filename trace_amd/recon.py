@@ -124,6 +124,25 @@ class Projector:
     def set_exact_sqrt(self, on: bool):
         check(lib().trace_set_option(self._h, _capi.TRACE_OPT_EXACT_SQRT, 1 if on else 0))
 
+    def set_backprojector(self, kind: str):
+        """'tile' (LDS-tiled, default) or 'atomic' (whole-ray global atomics)."""
+        v = {"tile": 0, "atomic": 1}[kind]
+        check(lib().trace_set_option(self._h, _capi.TRACE_OPT_BACKPROJECTOR, v))
+
+    def set_launch_shape(self, fwd_batch: int, tile_w: int, tile_h: int, tile_batch: int):
+        check(lib().trace_set_launch_shape(self._h, fwd_batch, tile_w, tile_h, tile_batch))
+
+    def launch_shape(self) -> tuple:
+        v = [ct.c_int() for _ in range(4)]
+        check(lib().trace_get_launch_shape(self._h, *[ct.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def segment_counts(self, out=None):
+        """Segments per pixel of one slice, [N][N] (float, integer-valued)."""
+        d = self._empty(self.N, self.N) if out is None else out
+        check(lib().trace_segment_counts(self._h, _dptr(d)))
+        return d
+
     def set_phase_timing(self, on: bool):
         check(lib().trace_set_option(self._h, _capi.TRACE_OPT_PHASE_TIMING, 1 if on else 0))
 
