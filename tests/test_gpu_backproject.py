@@ -58,7 +58,7 @@ def test_segment_counts_large(gpu):
         pj.set_backprojector("tile")
         t = pj.segment_counts().cpu().numpy()
         nnz, _ = pj.count_segments()
-    assert int(a.sum()) == nnz
+    assert int(a.astype(np.int64).sum()) == nnz
     assert np.array_equal(a, t)
 
 

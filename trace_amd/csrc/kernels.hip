@@ -147,21 +147,34 @@ __device__ __forceinline__ void tile_columns(const AngleInfo &a, float x0, float
   cmax = (int)floorf(tmax + 1.0f - yi0);
 }
 
-// Workgroup (tile, slice batch): LDS accumulators acc[TH*TW][SB], all
+// LDS accumulator layout of the tile backprojector: one plane per slice,
+// rows padded to TW + 2 floats.  Neighbouring rays of one angle sit one
+// pixel apart across their direction, so at a given step the lanes of a
+// wavefront hit pixels offset by (0, +-1), (+-1, 0) or (+-1, +-1): with a
+// pitch of TW + 2 those map to bank strides of 1, 2 and 3 instead of the
+// full conflict of an unpadded power-of-two pitch.
+template <int TW, int TH>
+struct TilePitch {
+  static constexpr int kPitch = TW + 2;
+  static constexpr int kPlane = TH * kPitch;
+};
+
+// Workgroup (tile, slice batch): LDS accumulators acc[SB][TH][TW + 2], all
 // angles; lanes take consecutive (angle, column) pairs of the flattened
 // per-angle column ranges.
 template <int TW, int TH, int SB, int MODE, bool kExact>
 __global__ void __launch_bounds__(kTileBlock)
 k_bp_tile(TileArgs a) {
-  extern __shared__ float acc[];  // [TH*TW][SB]
+  extern __shared__ float acc[];  // [SB][TH][TW + 2]
+  constexpr int PITCH = TilePitch<TW, TH>::kPitch;
+  constexpr int PLANE = TilePitch<TW, TH>::kPlane;
   const int N = a.g.N;
   const int tiles_x = (N + TW - 1) / TW;
   const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
   const int tx0 = tx * TW, ty0 = ty * TH;
   const int tx1 = min(tx0 + TW, N), ty1 = min(ty0 + TH, N);
   const int b = blockIdx.y;
-  const int nloc = TW * TH * SB;
-  for (int i = threadIdx.x; i < nloc; i += kTileBlock) acc[i] = 0.f;
+  for (int i = threadIdx.x; i < SB * PLANE; i += kTileBlock) acc[i] = 0.f;
   __syncthreads();
 
   const float x0 = grid_line(N, tx0), x1 = grid_line(N, tx1);
@@ -198,10 +211,11 @@ k_bp_tile(TileArgs a) {
                                   const unsigned lx = (unsigned)(ix - tx0);
                                   const unsigned ly = (unsigned)(iy - ty0);
                                   if (lx < (unsigned)(tx1 - tx0) && ly < (unsigned)(ty1 - ty0)) {
-                                    float *dst = acc + (ly * TW + lx) * SB;
+                                    float *dst = acc + ly * PITCH + lx;
                                     if (MODE == kBpSino) {
 #pragma unroll
-                                      for (int q = 0; q < SB; ++q) atomicAdd(dst + q, l * wv[q]);
+                                      for (int q = 0; q < SB; ++q)
+                                        atomicAdd(dst + q * PLANE, l * wv[q]);
                                     } else if (MODE == kBpOnes) {
                                       atomicAdd(dst, l);
                                     } else {
@@ -223,7 +237,7 @@ k_bp_tile(TileArgs a) {
     float *zs = a.z + (int64_t)s * N * N;
     for (int i = threadIdx.x; i < w_ * h_; i += kTileBlock) {
       const int ly = i / w_, lx = i - ly * w_;
-      zs[(int64_t)(ty0 + ly) * N + tx0 + lx] = acc[(ly * TW + lx) * SB + q];
+      zs[(int64_t)(ty0 + ly) * N + tx0 + lx] = acc[q * PLANE + ly * PITCH + lx];
     }
   }
 }
@@ -522,7 +536,7 @@ template <int TW, int TH, int SB, int MODE, bool E>
 hipError_t bp_launch(const TileArgs &a, int nbatch, hipStream_t st) {
   const int N = a.g.N;
   const int tiles = ((N + TW - 1) / TW) * ((N + TH - 1) / TH);
-  const size_t lds = (size_t)TW * TH * SB * sizeof(float);
+  const size_t lds = (size_t)SB * TilePitch<TW, TH>::kPlane * sizeof(float);
   auto kern = k_bp_tile<TW, TH, SB, MODE, E>;
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute((const void *)kern,
