@@ -150,7 +150,10 @@ struct trace_handle {
   float *xa = nullptr;         // [S+2][N][N] work image (PML/APMLR)
   float *xc = nullptr;         // [S+2][N][N]
   int *flag = nullptr;
-  unsigned long long *counters = nullptr;  // [2]
+  unsigned long long *counters = nullptr;  // [3]: segments, outside image, max length bits
+  unsigned *maxw = nullptr;    // [S] bits of max |w| per slice
+  float maxlen = 0.f;          // longest segment of the geometry
+  int64_t nnz = 0, nnz_oor = 0;
   PhaseTimer timer;
 };
 
@@ -164,10 +167,23 @@ struct trace_handle {
 
 namespace {
 
+// Ray table and segment statistics (count, longest segment) of the
+// geometry; one synchronous read-back per handle.
 int ensure_rays(trace_handle *h) {
   if (h->rays_ready) return TRACE_OK;
   if (!h->rays) HIP_TRY(dev_alloc(&h->rays, (size_t)h->geo.rays_per_slice()));
   TIMED(TRACE_PHASE_SETUP, launch_ray_info(h->geo, h->rays, h->stream));
+  HIP_TRY(hipMemsetAsync(h->counters, 0, 3 * sizeof(unsigned long long), h->stream));
+  TIMED(TRACE_PHASE_SETUP, launch_count_segments(h->geo, h->rays, h->counters, h->counters + 1,
+                                                 reinterpret_cast<unsigned *>(h->counters + 2),
+                                                 h->stream));
+  unsigned long long c[3] = {0, 0, 0};
+  HIP_TRY(hipMemcpyAsync(c, h->counters, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->nnz = (int64_t)c[0];
+  h->nnz_oor = (int64_t)c[1];
+  const unsigned bits = (unsigned)c[2];
+  std::memcpy(&h->maxlen, &bits, sizeof(float));
   h->rays_ready = true;
   return TRACE_OK;
 }
@@ -192,8 +208,13 @@ int backproject(trace_handle *h, int mode, const float *w, float *z, int phase) 
   if (int rc = ensure_rays(h)) return rc;
   if (h->backprojector == 0) {
     if (int rc = ensure_count(h)) return rc;
-    TIMED(phase, launch_backproject(h->geo, h->rays, h->count, h->tile, mode, h->exact, w, z,
-                                    h->stream));
+    if (mode == kBpOutSino) {
+      if (!h->maxw) HIP_TRY(dev_alloc(&h->maxw, (size_t)h->geo.S));
+      TIMED(phase, launch_fill(reinterpret_cast<float *>(h->maxw), h->geo.S, 0.f, h->stream));
+      TIMED(phase, launch_absmax_slices(w, h->geo.P, h->geo.S, h->geo.C, h->maxw, h->stream));
+    }
+    TIMED(phase, launch_backproject(h->geo, h->rays, h->count, h->tile, mode, h->exact, w,
+                                    h->maxw, h->maxlen, z, h->stream));
   } else {
     const int ns = (mode == kBpOutSino) ? h->geo.S : 1;
     TIMED(phase, launch_fill(z, (int64_t)ns * h->geo.NN(), 0.f, h->stream));
@@ -362,7 +383,7 @@ int trace_create(trace_handle_t **out, const trace_geom_t *g, int device,
       (e = dev_alloc(&geo.d_yi, (size_t)geo.C)) != hipSuccess ||
       (e = dev_alloc(&h->den, (size_t)geo.NN())) != hipSuccess ||
       (e = dev_alloc(&h->flag, 1)) != hipSuccess ||
-      (e = dev_alloc(&h->counters, 2)) != hipSuccess)
+      (e = dev_alloc(&h->counters, 3)) != hipSuccess)
     return cleanup_fail(fail(e == hipErrorOutOfMemory ? TRACE_E_NOMEM : TRACE_E_HIP,
                              std::string("allocation: ") + hipGetErrorString(e)));
   if ((e = hipMemcpy(geo.d_ang, h->ang.data(), h->ang.size() * sizeof(AngleInfo),
@@ -382,7 +403,7 @@ void trace_destroy(trace_handle_t *h) {
   h->timer.release();
   for (void *p : {(void *)h->geo.d_ang, (void *)h->geo.d_yi, (void *)h->rays, (void *)h->count,
                   (void *)h->den, (void *)h->xb, (void *)h->w, (void *)h->z, (void *)h->xa,
-                  (void *)h->xc, (void *)h->flag, (void *)h->counters})
+                  (void *)h->xc, (void *)h->flag, (void *)h->counters, (void *)h->maxw})
     (void)hipFree(p);
   delete h;
 }
@@ -623,13 +644,8 @@ int trace_count_segments(trace_handle_t *h, int64_t *nnz, int64_t *out_of_image)
   if (!h || !nnz) return fail(TRACE_E_INVALID, "null argument");
   HIP_TRY(hipSetDevice(h->device));
   if (int rc = ensure_rays(h)) return rc;
-  HIP_TRY(hipMemsetAsync(h->counters, 0, 2 * sizeof(unsigned long long), h->stream));
-  HIP_TRY(launch_count_segments(h->geo, h->rays, h->counters, h->counters + 1, h->stream));
-  unsigned long long c[2] = {0, 0};
-  HIP_TRY(hipMemcpyAsync(c, h->counters, sizeof(c), hipMemcpyDeviceToHost, h->stream));
-  HIP_TRY(hipStreamSynchronize(h->stream));
-  *nnz = (int64_t)c[0];
-  if (out_of_image) *out_of_image = (int64_t)c[1];
+  *nnz = h->nnz;
+  if (out_of_image) *out_of_image = h->nnz_oor;
   return TRACE_OK;
 }
 

@@ -58,7 +58,9 @@ hipError_t launch_forward(const Geometry &g, const RayInfo *rays, int sb, int mo
                           hipStream_t st);
 hipError_t launch_backproject(const Geometry &g, const RayInfo *rays, const int *count,
                               const TileShape &ts, int mode, bool exact, const float *w,
-                              float *z, hipStream_t st);
+                              const unsigned *maxw, float maxlen, float *z, hipStream_t st);
+hipError_t launch_absmax_slices(const float *w, int P, int S, int C, unsigned *maxw,
+                                hipStream_t st);
 hipError_t launch_pack(const float *src, int nsrc, int off, float fill, int64_t NN,
                        int nslots, int sb, float *xb, hipStream_t st);
 hipError_t launch_update_em(bool sirt, float *x, const float *z, const float *den, int S,
@@ -78,7 +80,7 @@ hipError_t launch_dump_rays(const Geometry &g, const RayInfo *rays, int n, const
                             float *len2, hipStream_t st, bool exact);
 hipError_t launch_count_segments(const Geometry &g, const RayInfo *rays,
                                  unsigned long long *total, unsigned long long *oor,
-                                 hipStream_t st);
+                                 unsigned *maxlen, hipStream_t st);
 hipError_t launch_check_finite(const float *v, int64_t n, int *flag, hipStream_t st);
 
 }  // namespace trace_hip

@@ -129,6 +129,8 @@ struct TileArgs {
   int P, S;
   float yi0;          // detector coordinate of column 0
   const float *w;     // [P][S][C] weights (kBpSino)
+  const unsigned *maxw;  // [S] bits of max |w| per slice (kBpSino)
+  float maxlen;       // longest segment of the geometry
   float *z;           // [S][N][N] output (kBpOnes/kBpCount: [N][N])
 };
 
@@ -148,24 +150,42 @@ __device__ __forceinline__ void tile_columns(const AngleInfo &a, float x0, float
 }
 
 // LDS accumulator layout of the tile backprojector: one plane per slice,
-// rows padded to TW + 2 floats.  Neighbouring rays of one angle sit one
-// pixel apart across their direction, so at a given step the lanes of a
-// wavefront hit pixels offset by (0, +-1), (+-1, 0) or (+-1, +-1): with a
-// pitch of TW + 2 those map to bank strides of 1, 2 and 3 instead of the
-// full conflict of an unpadded power-of-two pitch.
+// rows padded to TW + 1 64-bit words.  Neighbouring rays of one angle sit
+// one pixel apart across their direction, so at a given step the lanes of a
+// wavefront hit pixels offset by (0, +-1), (+-1, 0) or (+-1, +-1); the odd
+// pitch spreads those over the banks.
 template <int TW, int TH>
 struct TilePitch {
-  static constexpr int kPitch = TW + 2;
+  static constexpr int kPitch = TW + 1;
   static constexpr int kPlane = TH * kPitch;
 };
 
-// Workgroup (tile, slice batch): LDS accumulators acc[SB][TH][TW + 2], all
+// Fixed-point accumulation.  LDS float atomics (ds_add_f32) run at a small
+// fraction of the rate of LDS integer atomics on gfx950, so each term len*w
+// is scaled by 2^e (e per slice, such that |len*w| * 2^e < 2^30 for every
+// term: from the slice's largest |w| and the geometry's longest segment),
+// rounded to an integer and added with ds_add_u64.  Integer addition is
+// associative, so the result does not depend on the order in which lanes
+// and waves arrive: the backprojection is bit-reproducible, and its error
+// (at most half a unit of 2^-e per term) is far below that of an fp32 sum.
+// A slice whose weights are not all finite (y = 0 rays give inf/NaN in the
+// reference too) is accumulated in float instead, in the low word of the
+// same slot, so that inf/NaN propagate as they do in the reference.
+__device__ __forceinline__ int fixed_exponent(float maxw, float maxlen, bool &fixed) {
+  const float m = maxw * maxlen;
+  fixed = __builtin_isfinite(m);
+  if (!fixed) return 0;
+  if (m == 0.f) return 0;
+  return 29 - ilogbf(m);
+}
+
+// Workgroup (tile, slice batch): LDS accumulators acc[SB][TH][TW + 1], all
 // angles; lanes take consecutive (angle, column) pairs of the flattened
 // per-angle column ranges.
 template <int TW, int TH, int SB, int MODE, bool kExact>
 __global__ void __launch_bounds__(kTileBlock)
 k_bp_tile(TileArgs a) {
-  extern __shared__ float acc[];  // [SB][TH][TW + 2]
+  extern __shared__ unsigned long long acc[];  // [SB][TH][TW + 1]
   constexpr int PITCH = TilePitch<TW, TH>::kPitch;
   constexpr int PLANE = TilePitch<TW, TH>::kPlane;
   const int N = a.g.N;
@@ -174,7 +194,17 @@ k_bp_tile(TileArgs a) {
   const int tx0 = tx * TW, ty0 = ty * TH;
   const int tx1 = min(tx0 + TW, N), ty1 = min(ty0 + TH, N);
   const int b = blockIdx.y;
-  for (int i = threadIdx.x; i < SB * PLANE; i += kTileBlock) acc[i] = 0.f;
+  for (int i = threadIdx.x; i < SB * PLANE; i += kTileBlock) acc[i] = 0ull;
+  // per-slice scale exponents (uniform across the workgroup)
+  int ex[SB];
+  bool fx[SB];
+#pragma unroll
+  for (int q = 0; q < SB; ++q) {
+    const int s = b * SB + q;
+    float m = 1.0f;
+    if (MODE == kBpSino) m = (s < a.S) ? __uint_as_float(a.maxw[s]) : 0.f;
+    ex[q] = fixed_exponent(m, a.maxlen, fx[q]);
+  }
   __syncthreads();
 
   const float x0 = grid_line(N, tx0), x1 = grid_line(N, tx1);
@@ -211,15 +241,20 @@ k_bp_tile(TileArgs a) {
                                   const unsigned lx = (unsigned)(ix - tx0);
                                   const unsigned ly = (unsigned)(iy - ty0);
                                   if (lx < (unsigned)(tx1 - tx0) && ly < (unsigned)(ty1 - ty0)) {
-                                    float *dst = acc + ly * PITCH + lx;
-                                    if (MODE == kBpSino) {
-#pragma unroll
-                                      for (int q = 0; q < SB; ++q)
-                                        atomicAdd(dst + q * PLANE, l * wv[q]);
-                                    } else if (MODE == kBpOnes) {
-                                      atomicAdd(dst, l);
+                                    unsigned long long *dst = acc + ly * PITCH + lx;
+                                    if (MODE == kBpCount) {
+                                      atomicAdd(dst, 1ull);
                                     } else {
-                                      atomicAdd(dst, 1.0f);
+#pragma unroll
+                                      for (int q = 0; q < SB; ++q) {
+                                        const float t = (MODE == kBpSino) ? l * wv[q] : l;
+                                        if (fx[q]) {
+                                          const int ti = (int)rintf(ldexpf(t, ex[q]));
+                                          atomicAdd(dst + q * PLANE, (unsigned long long)(long long)ti);
+                                        } else {
+                                          atomicAdd(reinterpret_cast<float *>(dst + q * PLANE), t);
+                                        }
+                                      }
                                     }
                                   }
                                 });
@@ -231,15 +266,40 @@ k_bp_tile(TileArgs a) {
   __syncthreads();
   // coalesced write-out: plane by plane, row-contiguous
   const int w_ = tx1 - tx0, h_ = ty1 - ty0;
+#pragma unroll
   for (int q = 0; q < SB; ++q) {
     const int s = b * SB + q;
     if (MODE == kBpSino && s >= a.S) break;
     float *zs = a.z + (int64_t)s * N * N;
     for (int i = threadIdx.x; i < w_ * h_; i += kTileBlock) {
       const int ly = i / w_, lx = i - ly * w_;
-      zs[(int64_t)(ty0 + ly) * N + tx0 + lx] = acc[q * PLANE + ly * PITCH + lx];
+      const unsigned long long v = acc[q * PLANE + ly * PITCH + lx];
+      float out;
+      if (MODE == kBpCount)
+        out = (float)v;
+      else if (fx[q])
+        out = (float)ldexp((double)(long long)v, -ex[q]);
+      else
+        out = __uint_as_float((unsigned)v);
+      zs[(int64_t)(ty0 + ly) * N + tx0 + lx] = out;
     }
   }
+}
+
+// Per-slice maximum of |w| over w [P][S][C] (float bits, atomicMax on the
+// non-negative bit patterns; NaN sorts above inf).  maxw must be zeroed.
+__global__ void __launch_bounds__(kPixBlock)
+k_absmax_slices(const float *__restrict__ w, int P, int S, int C, unsigned *__restrict__ maxw) {
+  const int s = blockIdx.y;
+  const int64_t n = (int64_t)P * C;
+  unsigned m = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kPixBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kPixBlock) {
+    const int64_t p = i / C, c = i - p * C;
+    m = max(m, __float_as_uint(__builtin_fabsf(w[(p * S + s) * C + c])));
+  }
+  for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_down((int)m, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(maxw + s, m);
 }
 
 // ------------------------------------------------------- pack / update --
@@ -478,27 +538,34 @@ __global__ void k_dump_rays(GridInfo g, const AngleInfo *__restrict__ ang,
   counts[t] = cnt;
 }
 
+// Segment statistics of one slice: total count, count outside the image,
+// and the bits of the longest segment under either length rounding.
 __global__ void __launch_bounds__(kRayBlock)
 k_count_segments(GridInfo g, const AngleInfo *__restrict__ ang,
                  const RayInfo *__restrict__ rays, int P,
-                 unsigned long long *total, unsigned long long *oor) {
+                 unsigned long long *total, unsigned long long *oor, unsigned *maxlen) {
   const int64_t t = (int64_t)blockIdx.x * kRayBlock + threadIdx.x;
   const int C = g.C;
   unsigned cnt = 0, bad = 0;
+  float lmax = 0.f;
   if (t < (int64_t)P * C) {
     const int p = (int)(t / C);
     const unsigned NN = (unsigned)g.N * (unsigned)g.N;
-    cnt = traverse_full<true>(g, rays[t], ang[p].quadrant, [&](int ix, int iy, float, float) {
+    cnt = traverse_full<true>(g, rays[t], ang[p].quadrant, [&](int ix, int iy, float l, float l2) {
       bad += ((unsigned)(ix + iy * g.N) >= NN) ? 1u : 0u;
+      lmax = fmaxf(lmax, fmaxf(l, __builtin_amdgcn_sqrtf(l2)));
     });
   }
+  unsigned lb = __float_as_uint(lmax);
   for (int off = 32; off > 0; off >>= 1) {
     cnt += __shfl_down(cnt, off, 64);
     bad += __shfl_down(bad, off, 64);
+    lb = max(lb, (unsigned)__shfl_down((int)lb, off, 64));
   }
   if ((threadIdx.x & 63) == 0) {
     atomicAdd(total, (unsigned long long)cnt);
     if (bad) atomicAdd(oor, (unsigned long long)bad);
+    atomicMax(maxlen, lb);
   }
 }
 
@@ -536,7 +603,7 @@ template <int TW, int TH, int SB, int MODE, bool E>
 hipError_t bp_launch(const TileArgs &a, int nbatch, hipStream_t st) {
   const int N = a.g.N;
   const int tiles = ((N + TW - 1) / TW) * ((N + TH - 1) / TH);
-  const size_t lds = (size_t)SB * TilePitch<TW, TH>::kPlane * sizeof(float);
+  const size_t lds = (size_t)SB * TilePitch<TW, TH>::kPlane * sizeof(unsigned long long);
   auto kern = k_bp_tile<TW, TH, SB, MODE, E>;
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute((const void *)kern,
@@ -592,8 +659,10 @@ hipError_t launch_forward(const Geometry &g, const RayInfo *rays, int sb, int mo
 
 hipError_t launch_backproject(const Geometry &g, const RayInfo *rays, const int *count,
                               const TileShape &ts, int mode, bool exact, const float *w,
-                              float *z, hipStream_t st) {
+                              const unsigned *maxw, float maxlen, float *z, hipStream_t st) {
   TileArgs a;
+  a.maxw = maxw;
+  a.maxlen = maxlen;
   a.g = g.grid;
   a.ang = g.d_ang;
   a.rays = rays;
@@ -707,10 +776,19 @@ hipError_t launch_dump_rays(const Geometry &g, const RayInfo *rays, int n, const
 
 hipError_t launch_count_segments(const Geometry &g, const RayInfo *rays,
                                  unsigned long long *total, unsigned long long *oor,
-                                 hipStream_t st) {
+                                 unsigned *maxlen, hipStream_t st) {
   const int64_t n = g.rays_per_slice();
   hipLaunchKernelGGL(k_count_segments, dim3(blocks_for(n, kRayBlock)), dim3(kRayBlock), 0, st,
-                     g.grid, g.d_ang, rays, g.P, total, oor);
+                     g.grid, g.d_ang, rays, g.P, total, oor, maxlen);
+  return hipGetLastError();
+}
+
+hipError_t launch_absmax_slices(const float *w, int P, int S, int C, unsigned *maxw,
+                                hipStream_t st) {
+  int64_t n = (int64_t)P * C;
+  unsigned bx = blocks_for(n, kPixBlock);
+  if (bx > 512) bx = 512;
+  hipLaunchKernelGGL(k_absmax_slices, dim3(bx, S), dim3(kPixBlock), 0, st, w, P, S, C, maxw);
   return hipGetLastError();
 }
 
