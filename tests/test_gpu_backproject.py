@@ -12,8 +12,8 @@ from conftest import adversarial_angles, rel_l2
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(4, 64, 64, 4), (4, 32, 32, 4), (4, 64, 32, 4), (4, 32, 64, 4),
-          (8, 32, 32, 8), (8, 64, 32, 8), (8, 64, 64, 8), (16, 32, 32, 16)]
+SHAPES = [(16, 32, 32, 4), (4, 32, 32, 2), (8, 32, 32, 8), (4, 32, 16, 4), (4, 16, 32, 4),
+          (4, 16, 16, 4), (8, 16, 16, 8), (4, 64, 32, 4), (4, 64, 64, 2), (16, 32, 32, 16)]
 
 
 def _oracle_counts(oracle, th, N, center):

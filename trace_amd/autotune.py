@@ -15,8 +15,8 @@ import json
 import os
 
 FWD_BATCHES = (4, 8, 16)
-TILES = ((64, 64, 4), (32, 32, 4), (64, 32, 4), (32, 64, 4), (32, 32, 8),
-         (64, 32, 8), (64, 64, 8), (32, 32, 16))
+TILES = ((32, 32, 4), (32, 32, 2), (32, 32, 8), (32, 16, 4), (16, 32, 4),
+         (16, 16, 4), (16, 16, 8), (64, 32, 4), (64, 64, 2), (32, 32, 16))
 
 _CACHE: dict = {}
 

@@ -63,8 +63,9 @@ hipError_t dev_alloc(T **p, size_t count) {
 
 // Launch shapes compiled into the library (kernels.hip, launch_backproject).
 bool tile_shape_supported(int tw, int th, int sb) {
-  static const int shapes[][3] = {{64, 64, 4}, {32, 32, 4}, {64, 32, 4}, {32, 64, 4},
-                                  {32, 32, 8}, {64, 32, 8}, {64, 64, 8}, {32, 32, 16}};
+  static const int shapes[][3] = {{32, 32, 4}, {32, 32, 2}, {32, 32, 8}, {32, 16, 4},
+                                  {16, 32, 4}, {16, 16, 4}, {16, 16, 8}, {64, 32, 4},
+                                  {64, 64, 2}, {32, 32, 16}};
   for (auto &s : shapes)
     if (s[0] == tw && s[1] == th && s[2] == sb) return true;
   return false;
@@ -134,8 +135,8 @@ struct trace_handle {
   std::vector<AngleInfo> ang;
   bool exact = true;
   int backprojector = 0;  // 0 tile (LDS), 1 whole-ray global atomics
-  int fwd_sb = 4;         // slices per forward lane (xb layout)
-  TileShape tile;
+  int fwd_sb = 16;        // slices per forward lane (xb layout)
+  TileShape tile{32, 32, 4};
   RayInfo *rays = nullptr;     // [P][C]
   bool rays_ready = false;
   int *count = nullptr;        // [P] columns-per-tile bound for count_shape

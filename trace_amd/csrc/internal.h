@@ -30,7 +30,7 @@ struct Geometry {
 // Launch shape of the tile backprojector: tile of tw x th pixels, sb slices
 // per workgroup (LDS: tw*th*sb floats).
 struct TileShape {
-  int tw = 64, th = 64, sb = 4;
+  int tw = 32, th = 32, sb = 4;
 };
 
 // Regularisation weights for the PML/APMLR stencils, computed on the host

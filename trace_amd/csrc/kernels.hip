@@ -676,13 +676,15 @@ hipError_t launch_backproject(const Geometry &g, const RayInfo *rays, const int 
 #define TRACE_BP_CASE(TW_, TH_, SB_)                              \
   if (ts.tw == TW_ && ts.th == TH_ && ts.sb == SB_)               \
     return bp_mode<TW_, TH_, SB_>(mode, exact, a, nb, st);
-  TRACE_BP_CASE(64, 64, 4)
   TRACE_BP_CASE(32, 32, 4)
-  TRACE_BP_CASE(64, 32, 4)
-  TRACE_BP_CASE(32, 64, 4)
+  TRACE_BP_CASE(32, 32, 2)
   TRACE_BP_CASE(32, 32, 8)
-  TRACE_BP_CASE(64, 32, 8)
-  TRACE_BP_CASE(64, 64, 8)
+  TRACE_BP_CASE(32, 16, 4)
+  TRACE_BP_CASE(16, 32, 4)
+  TRACE_BP_CASE(16, 16, 4)
+  TRACE_BP_CASE(16, 16, 8)
+  TRACE_BP_CASE(64, 32, 4)
+  TRACE_BP_CASE(64, 64, 2)
   TRACE_BP_CASE(32, 32, 16)
 #undef TRACE_BP_CASE
   return hipErrorInvalidValue;
