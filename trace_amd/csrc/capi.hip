@@ -211,8 +211,10 @@ int backproject(trace_handle *h, int mode, const float *w, float *z, int phase) 
     if (int rc = ensure_count(h)) return rc;
     if (mode == kBpOutSino) {
       if (!h->maxw) HIP_TRY(dev_alloc(&h->maxw, (size_t)h->geo.S));
-      TIMED(phase, launch_fill(reinterpret_cast<float *>(h->maxw), h->geo.S, 0.f, h->stream));
-      TIMED(phase, launch_absmax_slices(w, h->geo.P, h->geo.S, h->geo.C, h->maxw, h->stream));
+      TIMED(TRACE_PHASE_SETUP,
+            launch_fill(reinterpret_cast<float *>(h->maxw), h->geo.S, 0.f, h->stream));
+      TIMED(TRACE_PHASE_SETUP,
+            launch_absmax_slices(w, h->geo.P, h->geo.S, h->geo.C, h->maxw, h->stream));
     }
     TIMED(phase, launch_backproject(h->geo, h->rays, h->count, h->tile, mode, h->exact, w,
                                     h->maxw, h->maxlen, z, h->stream));
