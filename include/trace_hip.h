@@ -140,7 +140,7 @@ int trace_reconstruct(trace_handle_t *h, trace_alg_t alg, int iters,
 
 /* As trace_reconstruct with host buffers; copies in and out over PCIe and
  * returns after the result is in `recon`.  This is the call a reference main
- * makes in place of its engine loop. */
+ * makes in place of its engine loop (sirt_main.cc:145-202). */
 int trace_reconstruct_host(trace_handle_t *h, trace_alg_t alg, int iters,
                            const float *sino, float *recon,
                            const trace_params_t *prm);
