@@ -117,7 +117,7 @@ void trace_destroy(trace_handle_t *h);
 
 int trace_set_option(trace_handle_t *h, int option, int value);
 
-/* Launch shape: slices per forward-projector lane (4, 8 or 16) and the
+/* Launch shape: slices per forward-projector lane (4, 8, 16 or 32) and the
  * backprojector's image tile (tile_w x tile_h pixels, tile_batch slices per
  * workgroup).  Only the shapes compiled into the library are accepted
  * (TRACE_E_UNSUPPORTED otherwise); trace_amd.autotune picks among them.

@@ -134,8 +134,8 @@ def test_nonfinite_sinogram_rejected(gpu):
             pj.reconstruct("mlem", 1, sino, x)
 
 
-@pytest.mark.parametrize("shape", [(4, 64, 64, 2), (8, 32, 32, 8), (16, 32, 32, 16),
-                                   (16, 16, 16, 4)])
+@pytest.mark.parametrize("shape", [(4, 64, 64, 2), (8, 32, 32, 8), (32, 16, 16, 16),
+                                   (16, 16, 16, 8), (16, 24, 24, 8)])
 @pytest.mark.parametrize("alg", ["sirt", "mlem"])
 def test_launch_shapes_recon(gpu, oracle, alg, shape):
     """Every compiled launch shape gives the same reconstruction."""

@@ -14,9 +14,9 @@ from __future__ import annotations
 import json
 import os
 
-FWD_BATCHES = (4, 8, 16)
-TILES = ((32, 32, 4), (32, 32, 2), (32, 32, 8), (32, 16, 4), (16, 32, 4),
-         (16, 16, 4), (16, 16, 8), (64, 32, 4), (64, 64, 2), (32, 32, 16))
+FWD_BATCHES = (4, 8, 16, 32)
+TILES = ((16, 16, 8), (16, 16, 16), (16, 16, 4), (24, 24, 8), (32, 16, 8),
+         (16, 32, 8), (32, 32, 4), (32, 32, 8), (8, 8, 16), (64, 64, 2))
 
 _CACHE: dict = {}
 

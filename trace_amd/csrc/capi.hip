@@ -63,9 +63,9 @@ hipError_t dev_alloc(T **p, size_t count) {
 
 // Launch shapes compiled into the library (kernels.hip, launch_backproject).
 bool tile_shape_supported(int tw, int th, int sb) {
-  static const int shapes[][3] = {{32, 32, 4}, {32, 32, 2}, {32, 32, 8}, {32, 16, 4},
-                                  {16, 32, 4}, {16, 16, 4}, {16, 16, 8}, {64, 32, 4},
-                                  {64, 64, 2}, {32, 32, 16}};
+  static const int shapes[][3] = {{16, 16, 8}, {16, 16, 16}, {16, 16, 4}, {24, 24, 8},
+                                  {32, 16, 8}, {16, 32, 8}, {32, 32, 4}, {32, 32, 8},
+                                  {8, 8, 16},  {64, 64, 2}};
   for (auto &s : shapes)
     if (s[0] == tw && s[1] == th && s[2] == sb) return true;
   return false;
@@ -136,7 +136,7 @@ struct trace_handle {
   bool exact = true;
   int backprojector = 0;  // 0 tile (LDS), 1 whole-ray global atomics
   int fwd_sb = 16;        // slices per forward lane (xb layout)
-  TileShape tile{32, 32, 4};
+  TileShape tile{16, 16, 8};
   RayInfo *rays = nullptr;     // [P][C]
   bool rays_ready = false;
   int *count = nullptr;        // [P] columns-per-tile bound for count_shape
@@ -146,7 +146,7 @@ struct trace_handle {
   bool den_exact = true;
   float *xb = nullptr;         // [ceil(S/fwd_sb)][N][N][fwd_sb]
   int xb_sb = 0;
-  float *w = nullptr;          // [P][S][C]
+  float *w = nullptr;          // [P][C][S] per-ray weights
   float *z = nullptr;          // [S][N][N]
   float *xa = nullptr;         // [S+2][N][N] work image (PML/APMLR)
   float *xc = nullptr;         // [S+2][N][N]
@@ -204,7 +204,7 @@ int ensure_count(trace_handle *h) {
 }
 
 // A^T applied with the configured backprojector.  mode: kBpOutSino (w
-// [P][S][C] -> z [S][N][N]), kBpOutOnes / kBpOutCount (-> z [N][N]).
+// [P][C][S] -> z [S][N][N]), kBpOutOnes / kBpOutCount (-> z [N][N]).
 int backproject(trace_handle *h, int mode, const float *w, float *z, int phase) {
   if (int rc = ensure_rays(h)) return rc;
   if (h->backprojector == 0) {
@@ -436,8 +436,8 @@ int trace_set_launch_shape(trace_handle_t *h, int fwd_batch, int tile_w, int til
                            int tile_batch) {
   g_err.clear();
   if (!h) return fail(TRACE_E_INVALID, "handle is null");
-  if (fwd_batch != 4 && fwd_batch != 8 && fwd_batch != 16)
-    return fail(TRACE_E_UNSUPPORTED, "forward slice batch must be 4, 8 or 16");
+  if (fwd_batch != 4 && fwd_batch != 8 && fwd_batch != 16 && fwd_batch != 32)
+    return fail(TRACE_E_UNSUPPORTED, "forward slice batch must be 4, 8, 16 or 32");
   if (!tile_shape_supported(tile_w, tile_h, tile_batch))
     return fail(TRACE_E_UNSUPPORTED, "tile shape not compiled into this library");
   h->fwd_sb = fwd_batch;
@@ -585,7 +585,9 @@ int trace_adjoint(trace_handle_t *h, const float *r, float *z) {
   if (!h) return fail(TRACE_E_INVALID, "handle is null");
   if (!r || !z) return fail(TRACE_E_INVALID, "null buffer");
   HIP_TRY(hipSetDevice(h->device));
-  return backproject(h, kBpOutSino, r, z, TRACE_PHASE_BACK);
+  if (int rc = ensure_work(h, false)) return rc;
+  TIMED(TRACE_PHASE_SETUP, launch_to_pcs(r, h->geo.P, h->geo.S, h->geo.C, h->w, h->stream));
+  return backproject(h, kBpOutSino, h->w, z, TRACE_PHASE_BACK);
 }
 
 int trace_column_sums(trace_handle_t *h, float *den) {

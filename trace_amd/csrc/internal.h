@@ -30,7 +30,7 @@ struct Geometry {
 // Launch shape of the tile backprojector: tile of tw x th pixels, sb slices
 // per workgroup (LDS: tw*th*sb floats).
 struct TileShape {
-  int tw = 32, th = 32, sb = 4;
+  int tw = 16, th = 16, sb = 8;
 };
 
 // Regularisation weights for the PML/APMLR stencils, computed on the host
@@ -61,6 +61,7 @@ hipError_t launch_backproject(const Geometry &g, const RayInfo *rays, const int 
                               const unsigned *maxw, float maxlen, float *z, hipStream_t st);
 hipError_t launch_absmax_slices(const float *w, int P, int S, int C, unsigned *maxw,
                                 hipStream_t st);
+hipError_t launch_to_pcs(const float *r, int P, int S, int C, float *w, hipStream_t st);
 hipError_t launch_pack(const float *src, int nsrc, int off, float fill, int64_t NN,
                        int nslots, int sb, float *xb, hipStream_t st);
 hipError_t launch_update_em(bool sirt, float *x, const float *z, const float *den, int S,

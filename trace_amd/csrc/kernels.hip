@@ -73,6 +73,7 @@ __device__ __forceinline__ void load_slices(const float *__restrict__ p, float (
 
 // One lane per ray (p, c) and slice batch b (blockIdx.y).  kFwdY writes
 // y[p][s][c]; kFwdSIRT / kFwdEM write the per-ray weight of UpdateReconReplica
+// to w[p][c][s] (slices innermost, read as a block by the backprojector)
 //   SIRT  w = (p - y)/a2, a2 = sum len^2    (sirt.cc:47-52)
 //   EM    w = p/y                           (mlem.cc:50, pml.cc:236)
 template <int SB, int MODE, bool kExact>
@@ -107,14 +108,27 @@ k_forward(GridInfo g, const AngleInfo *__restrict__ ang, const RayInfo *__restri
     const int s = b * SB + k;
     if (s >= S) break;
     const int64_t o = ((int64_t)p * S + s) * C + c;
+    const int64_t ow = t * S + s;
     if (MODE == kFwdY) {
       out[o] = acc[k];
     } else if (MODE == kFwdSIRT) {
-      out[o] = (sino[o] - acc[k]) / a2;
+      out[ow] = (sino[o] - acc[k]) / a2;
     } else {
-      out[o] = sino[o] / acc[k];
+      out[ow] = sino[o] / acc[k];
     }
   }
+}
+
+// w[p][c][s] = r[p][s][c]
+__global__ void __launch_bounds__(256)
+k_to_pcs(const float *__restrict__ r, int P, int S, int C, float *__restrict__ w) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;  // over [P][S][C]
+  if (i >= (int64_t)P * S * C) return;
+  const int c = (int)(i % C);
+  const int64_t ps = i / C;
+  const int s = (int)(ps % S);
+  const int64_t p = ps / S;
+  w[(p * C + c) * S + s] = r[i];
 }
 
 // ------------------------------------------------------- backprojector --
@@ -127,8 +141,9 @@ struct TileArgs {
   const RayInfo *rays;
   const int *count;   // [P] upper bound of the columns touching any tile
   int P, S;
+  int count_lds;      // 1: count[] is staged in LDS after the accumulators
   float yi0;          // detector coordinate of column 0
-  const float *w;     // [P][S][C] weights (kBpSino)
+  const float *w;     // [P][C][S] weights (kBpSino)
   const unsigned *maxw;  // [S] bits of max |w| per slice (kBpSino)
   float maxlen;       // longest segment of the geometry
   float *z;           // [S][N][N] output (kBpOnes/kBpCount: [N][N])
@@ -185,9 +200,11 @@ __device__ __forceinline__ int fixed_exponent(float maxw, float maxlen, bool &fi
 template <int TW, int TH, int SB, int MODE, bool kExact>
 __global__ void __launch_bounds__(kTileBlock)
 k_bp_tile(TileArgs a) {
-  extern __shared__ unsigned long long acc[];  // [SB][TH][TW + 1]
+  extern __shared__ unsigned long long acc[];  // [SB][TH][TW + 1], then int count[P]
   constexpr int PITCH = TilePitch<TW, TH>::kPitch;
   constexpr int PLANE = TilePitch<TW, TH>::kPlane;
+  int *lcount = reinterpret_cast<int *>(acc + SB * PLANE);
+  const int *count = a.count_lds ? lcount : a.count;
   const int N = a.g.N;
   const int tiles_x = (N + TW - 1) / TW;
   const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
@@ -195,6 +212,8 @@ k_bp_tile(TileArgs a) {
   const int tx1 = min(tx0 + TW, N), ty1 = min(ty0 + TH, N);
   const int b = blockIdx.y;
   for (int i = threadIdx.x; i < SB * PLANE; i += kTileBlock) acc[i] = 0ull;
+  if (a.count_lds)
+    for (int i = threadIdx.x; i < a.P; i += kTileBlock) lcount[i] = a.count[i];
   // per-slice scale exponents (uniform across the workgroup)
   int ex[SB];
   bool fx[SB];
@@ -212,7 +231,7 @@ k_bp_tile(TileArgs a) {
   const int C = a.g.C;
   // lane t starts at flat pair t and advances by kTileBlock
   int p = 0, k = threadIdx.x;
-  while (p < a.P && k >= a.count[p]) { k -= a.count[p]; ++p; }
+  while (p < a.P && k >= count[p]) { k -= count[p]; ++p; }
   while (p < a.P) {
     int cmin, cmax;
     const AngleInfo ang = a.ang[p];
@@ -223,10 +242,10 @@ k_bp_tile(TileArgs a) {
       bool any = true;
       if (MODE == kBpSino) {
         any = false;
+        const float *wr = a.w + ((int64_t)p * C + c) * a.S + b * SB;
 #pragma unroll
         for (int q = 0; q < SB; ++q) {
-          const int s = b * SB + q;
-          wv[q] = (s < a.S) ? a.w[((int64_t)p * a.S + s) * C + c] : 0.f;
+          wv[q] = (b * SB + q < a.S) ? wr[q] : 0.f;
           any |= (wv[q] != 0.f);
         }
       }
@@ -261,7 +280,7 @@ k_bp_tile(TileArgs a) {
       }
     }
     k += kTileBlock;
-    while (p < a.P && k >= a.count[p]) { k -= a.count[p]; ++p; }
+    while (p < a.P && k >= count[p]) { k -= count[p]; ++p; }
   }
   __syncthreads();
   // coalesced write-out: plane by plane, row-contiguous
@@ -286,7 +305,7 @@ k_bp_tile(TileArgs a) {
   }
 }
 
-// Per-slice maximum of |w| over w [P][S][C] (float bits, atomicMax on the
+// Per-slice maximum of |w| over w [P][C][S] (float bits, atomicMax on the
 // non-negative bit patterns; NaN sorts above inf).  maxw must be zeroed.
 __global__ void __launch_bounds__(kPixBlock)
 k_absmax_slices(const float *__restrict__ w, int P, int S, int C, unsigned *__restrict__ maxw) {
@@ -294,10 +313,8 @@ k_absmax_slices(const float *__restrict__ w, int P, int S, int C, unsigned *__re
   const int64_t n = (int64_t)P * C;
   unsigned m = 0;
   for (int64_t i = (int64_t)blockIdx.x * kPixBlock + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kPixBlock) {
-    const int64_t p = i / C, c = i - p * C;
-    m = max(m, __float_as_uint(__builtin_fabsf(w[(p * S + s) * C + c])));
-  }
+       i += (int64_t)gridDim.x * kPixBlock)
+    m = max(m, __float_as_uint(__builtin_fabsf(w[i * S + s])));
   for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_down((int)m, off, 64));
   if ((threadIdx.x & 63) == 0) atomicMax(maxw + s, m);
 }
@@ -505,7 +522,7 @@ k_adjoint_atomic(GridInfo g, const AngleInfo *__restrict__ ang,
   const unsigned NN = (unsigned)g.N * (unsigned)g.N;
   float w = 1.f;
   if (mode == kBpSino) {
-    w = r[((int64_t)p * S + s) * C + c];
+    w = r[t * S + s];  // [P][C][S]
     if (w == 0.f) return;
   }
   float *zs = z + (int64_t)s * NN;
@@ -599,11 +616,16 @@ hipError_t fwd_mode(int mode, const Geometry &g, const RayInfo *rays, const floa
   return hipGetLastError();
 }
 
+constexpr size_t kLdsBytes = 160 * 1024;
+
 template <int TW, int TH, int SB, int MODE, bool E>
-hipError_t bp_launch(const TileArgs &a, int nbatch, hipStream_t st) {
+hipError_t bp_launch(TileArgs a, int nbatch, hipStream_t st) {
   const int N = a.g.N;
   const int tiles = ((N + TW - 1) / TW) * ((N + TH - 1) / TH);
-  const size_t lds = (size_t)SB * TilePitch<TW, TH>::kPlane * sizeof(unsigned long long);
+  size_t lds = (size_t)SB * TilePitch<TW, TH>::kPlane * sizeof(unsigned long long);
+  // stage the per-angle column counts in LDS when they fit
+  a.count_lds = (lds + (size_t)a.P * sizeof(int) <= kLdsBytes / 2) ? 1 : 0;
+  if (a.count_lds) lds += (size_t)a.P * sizeof(int);
   auto kern = k_bp_tile<TW, TH, SB, MODE, E>;
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute((const void *)kern,
@@ -653,6 +675,8 @@ hipError_t launch_forward(const Geometry &g, const RayInfo *rays, int sb, int mo
                          : fwd_mode<8, false>(mode, g, rays, xb, sino, out, st);
     case 16: return exact ? fwd_mode<16, true>(mode, g, rays, xb, sino, out, st)
                           : fwd_mode<16, false>(mode, g, rays, xb, sino, out, st);
+    case 32: return exact ? fwd_mode<32, true>(mode, g, rays, xb, sino, out, st)
+                          : fwd_mode<32, false>(mode, g, rays, xb, sino, out, st);
     default: return hipErrorInvalidValue;
   }
 }
@@ -676,16 +700,16 @@ hipError_t launch_backproject(const Geometry &g, const RayInfo *rays, const int 
 #define TRACE_BP_CASE(TW_, TH_, SB_)                              \
   if (ts.tw == TW_ && ts.th == TH_ && ts.sb == SB_)               \
     return bp_mode<TW_, TH_, SB_>(mode, exact, a, nb, st);
-  TRACE_BP_CASE(32, 32, 4)
-  TRACE_BP_CASE(32, 32, 2)
-  TRACE_BP_CASE(32, 32, 8)
-  TRACE_BP_CASE(32, 16, 4)
-  TRACE_BP_CASE(16, 32, 4)
-  TRACE_BP_CASE(16, 16, 4)
   TRACE_BP_CASE(16, 16, 8)
-  TRACE_BP_CASE(64, 32, 4)
+  TRACE_BP_CASE(16, 16, 16)
+  TRACE_BP_CASE(16, 16, 4)
+  TRACE_BP_CASE(24, 24, 8)
+  TRACE_BP_CASE(32, 16, 8)
+  TRACE_BP_CASE(16, 32, 8)
+  TRACE_BP_CASE(32, 32, 4)
+  TRACE_BP_CASE(32, 32, 8)
+  TRACE_BP_CASE(8, 8, 16)
   TRACE_BP_CASE(64, 64, 2)
-  TRACE_BP_CASE(32, 32, 16)
 #undef TRACE_BP_CASE
   return hipErrorInvalidValue;
 }
@@ -698,6 +722,7 @@ hipError_t launch_pack(const float *src, int nsrc, int off, float fill, int64_t 
     case 4: hipLaunchKernelGGL(k_pack<4>, grid, block, 0, st, src, nsrc, off, fill, NN, nslots, xb); break;
     case 8: hipLaunchKernelGGL(k_pack<8>, grid, block, 0, st, src, nsrc, off, fill, NN, nslots, xb); break;
     case 16: hipLaunchKernelGGL(k_pack<16>, grid, block, 0, st, src, nsrc, off, fill, NN, nslots, xb); break;
+    case 32: hipLaunchKernelGGL(k_pack<32>, grid, block, 0, st, src, nsrc, off, fill, NN, nslots, xb); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -718,6 +743,7 @@ hipError_t launch_update_em(bool sirt, float *x, const float *z, const float *de
     TRACE_UPD_CASE(4)
     TRACE_UPD_CASE(8)
     TRACE_UPD_CASE(16)
+    TRACE_UPD_CASE(32)
     default: return hipErrorInvalidValue;
   }
 #undef TRACE_UPD_CASE
@@ -782,6 +808,12 @@ hipError_t launch_count_segments(const Geometry &g, const RayInfo *rays,
   const int64_t n = g.rays_per_slice();
   hipLaunchKernelGGL(k_count_segments, dim3(blocks_for(n, kRayBlock)), dim3(kRayBlock), 0, st,
                      g.grid, g.d_ang, rays, g.P, total, oor, maxlen);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_pcs(const float *r, int P, int S, int C, float *w, hipStream_t st) {
+  const int64_t n = (int64_t)P * S * C;
+  hipLaunchKernelGGL(k_to_pcs, dim3(blocks_for(n, 256)), dim3(256), 0, st, r, P, S, C, w);
   return hipGetLastError();
 }
 

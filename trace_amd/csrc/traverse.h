@@ -250,7 +250,7 @@ __device__ __forceinline__ int traverse_full(const GridInfo &g, const RayInfo &r
 // Rays whose merged points are used for a window of grid lines: every
 // segment whose midpoint falls in the lines' span [lo, hi] is produced,
 // together with a few neighbours (the callers filter by pixel).
-constexpr int kWindowMargin = 3;
+constexpr int kWindowMargin = 2;
 
 // Segments of the ray that can have their midpoint in pixel columns
 // [tx0, tx1) x rows [ty0, ty1): the ray is resumed just after the last A (or
