@@ -96,6 +96,9 @@ typedef struct trace_handle trace_handle_t;
 #define TRACE_OPT_BACKPROJECTOR 3 /* 0 (default): LDS-tiled backprojector;
                                      1: whole-ray scatter with global float
                                      atomics (an independent check) */
+#define TRACE_OPT_FWD_BUNDLE 4 /* adjacent angles per forward-projector
+                                  wavefront (1, 2, 4, 8, 16; the 64 lanes are
+                                  angles x 64/angles adjacent columns) */
 
 /* Kernel phases reported by trace_phase_times. */
 #define TRACE_PHASE_FORWARD 0 /* forward projection and per-ray weights */

@@ -145,3 +145,18 @@ def test_nonfinite_weights_propagate(gpu):
     assert np.isfinite(z[0]).all() and np.isfinite(z[2]).all()
     fin = np.isfinite(za)
     assert rel_l2(z[fin], za[fin]) < 1e-6
+
+
+@pytest.mark.parametrize("ba", [1, 2, 4, 8, 16])
+def test_forward_bundles_bitexact(gpu, oracle, ba):
+    """Ray bundles across adjacent angles change only the lane mapping."""
+    torch = gpu
+    from trace_amd import Projector
+    N, P, S = 70, 37, 5
+    rng = np.random.default_rng(ba)
+    th = rng.uniform(-1, 4, P).astype(np.float32)
+    x = rng.uniform(0, 1, (S, N, N)).astype(np.float32)
+    with Projector(th, S, N, N / 2) as pj:
+        pj.set_fwd_bundle(ba)
+        y = pj.forward(torch.from_numpy(x).cuda()).cpu().numpy()
+    assert np.array_equal(y.view(np.int32), oracle.forward(th, x, N / 2).view(np.int32))

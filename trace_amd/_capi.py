@@ -26,6 +26,7 @@ ALGS = {"sirt": TRACE_SIRT, "mlem": TRACE_MLEM, "pml": TRACE_PML, "apmlr": TRACE
 TRACE_OPT_EXACT_SQRT = 1
 TRACE_OPT_PHASE_TIMING = 2
 TRACE_OPT_BACKPROJECTOR = 3
+TRACE_OPT_FWD_BUNDLE = 4
 PHASES = ("forward", "back", "update", "setup")
 
 

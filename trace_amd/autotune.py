@@ -15,6 +15,7 @@ import json
 import os
 
 FWD_BATCHES = (4, 8, 16, 32)
+BUNDLES = (1, 2, 4, 8, 16)
 TILES = ((16, 16, 8), (16, 16, 16), (16, 16, 4), (24, 24, 8), (32, 16, 8),
          (16, 32, 8), (32, 32, 4), (32, 32, 8), (8, 8, 16), (64, 64, 2))
 
@@ -28,11 +29,11 @@ def _key(pj, alg):
 
 
 def autotune(pj, sino, recon, alg: str = "mlem", params=None,
-             fwd_batches=FWD_BATCHES, tiles=TILES, cache_file: str | None = None,
-             verbose: bool = False) -> dict:
+             fwd_batches=FWD_BATCHES, bundles=BUNDLES, tiles=TILES,
+             cache_file: str | None = None, verbose: bool = False) -> dict:
     """Times the candidates on (sino, a copy of recon) and sets the best
     shape on projector `pj`.  Returns {"shape": (fb, tw, th, tb),
-    "forward_ms": {...}, "back_ms": {...}}."""
+    "bundle": angles, "forward_ms": {...}, "back_ms": {...}}."""
     import torch
     key = _key(pj, alg)
     if cache_file and not _CACHE and os.path.exists(cache_file):
@@ -44,6 +45,7 @@ def autotune(pj, sino, recon, alg: str = "mlem", params=None,
     if key in _CACHE:
         res = _CACHE[key]
         pj.set_launch_shape(*res["shape"])
+        pj.set_fwd_bundle(res["bundle"])
         return res
     work = recon.clone()
     fwd_ms, back_ms = {}, {}
@@ -51,16 +53,22 @@ def autotune(pj, sino, recon, alg: str = "mlem", params=None,
     pj.set_phase_timing(True)
     try:
         for fb in fwd_batches:
-            pj.set_launch_shape(fb, *base_tile)
-            work.copy_(recon)
-            pj.reconstruct(alg, 1, sino, work, params)  # allocation, warm caches
-            pj.phase_times(reset=True)
-            work.copy_(recon)
-            pj.reconstruct(alg, 1, sino, work, params)
-            fwd_ms[fb] = pj.phase_times(reset=True)["forward"][0]
-            if verbose:
-                print(f"[autotune] forward batch {fb}: {fwd_ms[fb]:.2f} ms", flush=True)
-        best_fb = min(fwd_ms, key=fwd_ms.get)
+            if fb > max(4, pj.S):
+                continue  # all but one batch would be padding
+            for ba in bundles:
+                pj.set_launch_shape(fb, *base_tile)
+                pj.set_fwd_bundle(ba)
+                work.copy_(recon)
+                pj.reconstruct(alg, 1, sino, work, params)  # allocation, warm caches
+                pj.phase_times(reset=True)
+                work.copy_(recon)
+                pj.reconstruct(alg, 1, sino, work, params)
+                fwd_ms[(fb, ba)] = pj.phase_times(reset=True)["forward"][0]
+                if verbose:
+                    print(f"[autotune] forward batch {fb} bundle {ba}: "
+                          f"{fwd_ms[(fb, ba)]:.2f} ms", flush=True)
+        best_fb, best_ba = min(fwd_ms, key=fwd_ms.get)
+        pj.set_fwd_bundle(best_ba)
         for t in tiles:
             pj.set_launch_shape(best_fb, *t)
             work.copy_(recon)
@@ -74,8 +82,8 @@ def autotune(pj, sino, recon, alg: str = "mlem", params=None,
         torch.cuda.synchronize(pj.device)
     shape = (best_fb, *best_tile)
     pj.set_launch_shape(*shape)
-    res = {"shape": list(shape),
-           "forward_ms": {str(k): v for k, v in fwd_ms.items()},
+    res = {"shape": list(shape), "bundle": best_ba,
+           "forward_ms": {f"{k[0]}x{k[1]}": v for k, v in fwd_ms.items()},
            "back_ms": {"x".join(map(str, k)): v for k, v in back_ms.items()}}
     _CACHE[key] = res
     if cache_file:

@@ -136,6 +136,7 @@ struct trace_handle {
   bool exact = true;
   int backprojector = 0;  // 0 tile (LDS), 1 whole-ray global atomics
   int fwd_sb = 16;        // slices per forward lane (xb layout)
+  int fwd_bundle = 1;     // adjacent angles per forward wavefront
   TileShape tile{16, 16, 8};
   RayInfo *rays = nullptr;     // [P][C]
   bool rays_ready = false;
@@ -204,18 +205,13 @@ int ensure_count(trace_handle *h) {
 }
 
 // A^T applied with the configured backprojector.  mode: kBpOutSino (w
-// [P][C][S] -> z [S][N][N]), kBpOutOnes / kBpOutCount (-> z [N][N]).
+// [P][C][S] -> z [S][N][N], with h->maxw holding max |w| per slice, as left
+// by the forward projector or launch_to_pcs), kBpOutOnes / kBpOutCount
+// (-> z [N][N]).
 int backproject(trace_handle *h, int mode, const float *w, float *z, int phase) {
   if (int rc = ensure_rays(h)) return rc;
   if (h->backprojector == 0) {
     if (int rc = ensure_count(h)) return rc;
-    if (mode == kBpOutSino) {
-      if (!h->maxw) HIP_TRY(dev_alloc(&h->maxw, (size_t)h->geo.S));
-      TIMED(TRACE_PHASE_SETUP,
-            launch_fill(reinterpret_cast<float *>(h->maxw), h->geo.S, 0.f, h->stream));
-      TIMED(TRACE_PHASE_SETUP,
-            launch_absmax_slices(w, h->geo.P, h->geo.S, h->geo.C, h->maxw, h->stream));
-    }
     TIMED(phase, launch_backproject(h->geo, h->rays, h->count, h->tile, mode, h->exact, w,
                                     h->maxw, h->maxlen, z, h->stream));
   } else {
@@ -236,6 +232,17 @@ int ensure_den(trace_handle *h) {
 
 int nbatch(int S, int sb) { return (S + sb - 1) / sb; }
 
+// Forward projection of xb into weights w (mode kFwdOutSIRT/EM) or y
+// (kFwdOutY), with max |w| per slice folded into h->maxw.
+int forward(trace_handle *h, int mode, const float *sino, float *out) {
+  if (mode != kFwdOutY)
+    TIMED(TRACE_PHASE_SETUP,
+          launch_fill(reinterpret_cast<float *>(h->maxw), h->geo.S, 0.f, h->stream));
+  TIMED(TRACE_PHASE_FORWARD, launch_forward(h->geo, h->rays, h->fwd_sb, h->fwd_bundle, mode,
+                                            h->exact, h->xb, sino, out, h->maxw, h->stream));
+  return TRACE_OK;
+}
+
 int ensure_work(trace_handle *h, bool images) {
   const size_t nn = (size_t)h->geo.NN();
   const size_t S = (size_t)h->geo.S;
@@ -248,6 +255,7 @@ int ensure_work(trace_handle *h, bool images) {
     h->xb_sb = h->fwd_sb;
   }
   if (!h->w) HIP_TRY(dev_alloc(&h->w, (size_t)h->geo.P * S * h->geo.C));
+  if (!h->maxw) HIP_TRY(dev_alloc(&h->maxw, S));
   if (!h->z) HIP_TRY(dev_alloc(&h->z, S * nn));
   if (images) {
     if (!h->xa) HIP_TRY(dev_alloc(&h->xa, (S + 2) * nn));
@@ -337,6 +345,7 @@ int trace_create(trace_handle_t **out, const trace_geom_t *g, int device,
   if (g->num_grids != g->num_cols)
     return fail(TRACE_E_INVALID, "num_grids must equal num_cols (sirt.cc:182)");
   if (g->num_cols > 32768) return fail(TRACE_E_INVALID, "num_cols above 32768");
+  if (g->num_slices > 65535) return fail(TRACE_E_INVALID, "num_slices above 65535");
   if (!std::isfinite(g->center)) return fail(TRACE_E_INVALID, "center is not finite");
   if (device < 0) return fail(TRACE_E_INVALID, "device index is negative");
   for (int p = 0; p < g->num_projs; ++p)
@@ -427,6 +436,11 @@ int trace_set_option(trace_handle_t *h, int option, int value) {
       if (value != 0 && value != 1) return fail(TRACE_E_INVALID, "value must be 0 or 1");
       h->backprojector = value;
       return TRACE_OK;
+    case TRACE_OPT_FWD_BUNDLE:
+      if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
+        return fail(TRACE_E_INVALID, "angles per bundle must be 1, 2, 4, 8 or 16");
+      h->fwd_bundle = value;
+      return TRACE_OK;
     default:
       return fail(TRACE_E_UNSUPPORTED, "unknown option");
   }
@@ -482,8 +496,7 @@ int trace_reconstruct(trace_handle_t *h, trace_alg_t alg, int iters,
   if (alg == TRACE_SIRT || alg == TRACE_MLEM) {
     TIMED(TRACE_PHASE_SETUP, launch_pack(recon, S, 0, 0.f, NN, nslots, sb, h->xb, st));
     for (int it = 0; it < iters; ++it) {
-      TIMED(TRACE_PHASE_FORWARD,
-            launch_forward(g, h->rays, sb, fmode, h->exact, h->xb, sino, h->w, st));
+      if (int rc = forward(h, fmode, sino, h->w)) return rc;
       if (int rc = backproject(h, kBpOutSino, h->w, h->z, TRACE_PHASE_BACK)) return rc;
       TIMED(TRACE_PHASE_UPDATE,
             launch_update_em(alg == TRACE_SIRT, recon, h->z, h->den, S, NN, sb, h->xb, st));
@@ -494,8 +507,7 @@ int trace_reconstruct(trace_handle_t *h, trace_alg_t alg, int iters,
     float *cur = recon, *alt = h->xa;
     TIMED(TRACE_PHASE_SETUP, launch_pack(cur, S, 0, 0.f, NN, nslots, sb, h->xb, st));
     for (int it = 0; it < iters; ++it) {
-      TIMED(TRACE_PHASE_FORWARD,
-            launch_forward(g, h->rays, sb, fmode, h->exact, h->xb, sino, h->w, st));
+      if (int rc = forward(h, fmode, sino, h->w)) return rc;
       if (int rc = backproject(h, kBpOutSino, h->w, h->z, TRACE_PHASE_BACK)) return rc;
       TIMED(TRACE_PHASE_UPDATE, launch_update_pml(cur, alt, h->z, h->den, S, g.N, prm->beta,
                                                   st, nullptr, nullptr));
@@ -519,8 +531,7 @@ int trace_reconstruct(trace_handle_t *h, trace_alg_t alg, int iters,
   HIP_TRY(hipMemcpyAsync(pa + NN, recon, sizeof(float) * S * NN, hipMemcpyDeviceToDevice, st));
   TIMED(TRACE_PHASE_SETUP, launch_pack(pa, S + 2, off, 0.f, NN, nslots, sb, h->xb, st));
   for (int it = 0; it < iters; ++it) {
-    TIMED(TRACE_PHASE_FORWARD,
-          launch_forward(g, h->rays, sb, kFwdOutEM, h->exact, h->xb, sino, h->w, st));
+    if (int rc = forward(h, kFwdOutEM, sino, h->w)) return rc;
     if (int rc = backproject(h, kBpOutSino, h->w, h->z, TRACE_PHASE_BACK)) return rc;
     TIMED(TRACE_PHASE_UPDATE, launch_update_apmlr(pa, pb, h->z, h->den, S, g.N, rp, w,
                                                   prm->apmlr_compat, st, nullptr, nullptr));
@@ -575,9 +586,7 @@ int trace_forward(trace_handle_t *h, const float *x, float *y) {
   const int sb = h->fwd_sb;
   TIMED(TRACE_PHASE_SETUP,
         launch_pack(x, g.S, 0, 0.f, g.NN(), nbatch(g.S, sb) * sb, sb, h->xb, h->stream));
-  TIMED(TRACE_PHASE_FORWARD,
-        launch_forward(g, h->rays, sb, kFwdOutY, h->exact, h->xb, nullptr, y, h->stream));
-  return TRACE_OK;
+  return forward(h, kFwdOutY, nullptr, y);
 }
 
 int trace_adjoint(trace_handle_t *h, const float *r, float *z) {
@@ -586,7 +595,10 @@ int trace_adjoint(trace_handle_t *h, const float *r, float *z) {
   if (!r || !z) return fail(TRACE_E_INVALID, "null buffer");
   HIP_TRY(hipSetDevice(h->device));
   if (int rc = ensure_work(h, false)) return rc;
-  TIMED(TRACE_PHASE_SETUP, launch_to_pcs(r, h->geo.P, h->geo.S, h->geo.C, h->w, h->stream));
+  TIMED(TRACE_PHASE_SETUP,
+        launch_fill(reinterpret_cast<float *>(h->maxw), h->geo.S, 0.f, h->stream));
+  TIMED(TRACE_PHASE_SETUP,
+        launch_to_pcs(r, h->geo.P, h->geo.S, h->geo.C, h->w, h->maxw, h->stream));
   return backproject(h, kBpOutSino, h->w, z, TRACE_PHASE_BACK);
 }
 

@@ -53,15 +53,16 @@ int tile_count_bound(float sinq, float cosq, int TW, int TH);
 
 // Launchers (kernels.hip).  All are asynchronous on `st`.
 hipError_t launch_ray_info(const Geometry &g, RayInfo *rays, hipStream_t st);
-hipError_t launch_forward(const Geometry &g, const RayInfo *rays, int sb, int mode,
-                          bool exact, const float *xb, const float *sino, float *out,
-                          hipStream_t st);
+hipError_t launch_forward(const Geometry &g, const RayInfo *rays, int sb, int bundle_angles,
+                          int mode, bool exact, const float *xb, const float *sino,
+                          float *out, unsigned *maxw, hipStream_t st);
 hipError_t launch_backproject(const Geometry &g, const RayInfo *rays, const int *count,
                               const TileShape &ts, int mode, bool exact, const float *w,
                               const unsigned *maxw, float maxlen, float *z, hipStream_t st);
 hipError_t launch_absmax_slices(const float *w, int P, int S, int C, unsigned *maxw,
                                 hipStream_t st);
-hipError_t launch_to_pcs(const float *r, int P, int S, int C, float *w, hipStream_t st);
+hipError_t launch_to_pcs(const float *r, int P, int S, int C, float *w, unsigned *maxw,
+                         hipStream_t st);
 hipError_t launch_pack(const float *src, int nsrc, int off, float fill, int64_t NN,
                        int nslots, int sb, float *xb, hipStream_t st);
 hipError_t launch_update_em(bool sirt, float *x, const float *z, const float *den, int S,

@@ -71,20 +71,29 @@ __device__ __forceinline__ void load_slices(const float *__restrict__ p, float (
   }
 }
 
-// One lane per ray (p, c) and slice batch b (blockIdx.y).  kFwdY writes
-// y[p][s][c]; kFwdSIRT / kFwdEM write the per-ray weight of UpdateReconReplica
-// to w[p][c][s] (slices innermost, read as a block by the backprojector)
+// Lanes are grouped in ray bundles: a wavefront holds BA adjacent angles x
+// (64 / BA) adjacent detector columns (runtime `ba`), so at every step its
+// lanes sit on nearby pixels of nearby rays and share cache lines.  One lane
+// per ray (p, c) and slice batch b (blockIdx.y).  kFwdY writes y[p][s][c];
+// kFwdSIRT / kFwdEM write the per-ray weight of UpdateReconReplica
 //   SIRT  w = (p - y)/a2, a2 = sum len^2    (sirt.cc:47-52)
 //   EM    w = p/y                           (mlem.cc:50, pml.cc:236)
+// to w[p][c][s] (slices innermost, read as a block by the backprojector) and
+// fold max |w| per slice into maxw (bits; zeroed beforehand).
 template <int SB, int MODE, bool kExact>
 __global__ void __launch_bounds__(kRayBlock)
 k_forward(GridInfo g, const AngleInfo *__restrict__ ang, const RayInfo *__restrict__ rays,
-          int P, int S, const float *__restrict__ xb, const float *__restrict__ sino,
-          float *__restrict__ out) {
-  const int64_t t = (int64_t)blockIdx.x * kRayBlock + threadIdx.x;
+          int P, int S, int ba_log2, const float *__restrict__ xb,
+          const float *__restrict__ sino, float *__restrict__ out,
+          unsigned *__restrict__ maxw) {
   const int C = g.C;
-  if (t >= (int64_t)P * C) return;
-  const int c = (int)(t % C), p = (int)(t / C);
+  const int lane = threadIdx.x & 63;
+  const int bc_log2 = 6 - ba_log2;  // columns per bundle = 64 / BA
+  const int nbc = (C + (1 << bc_log2) - 1) >> bc_log2;
+  const int64_t bundle = (int64_t)blockIdx.x * (kRayBlock / 64) + (threadIdx.x >> 6);
+  const int p = (int)(bundle / nbc) * (1 << ba_log2) + (lane >> bc_log2);
+  const int c = (int)(bundle % nbc) * (1 << bc_log2) + (lane & ((1 << bc_log2) - 1));
+  const bool valid = (p < P) && (c < C);
   const int b = blockIdx.y;
   const unsigned NN = (unsigned)g.N * (unsigned)g.N;
   const float *xbb = xb + (int64_t)b * NN * SB;
@@ -92,43 +101,58 @@ k_forward(GridInfo g, const AngleInfo *__restrict__ ang, const RayInfo *__restri
 #pragma unroll
   for (int k = 0; k < SB; ++k) acc[k] = 0.f;
   float a2 = 0.f;
-  const RayInfo r = rays[t];
-  traverse_full<kExact>(g, r, ang[p].quadrant, [&](int ix, int iy, float l, float l2) {
-    const unsigned idx = (unsigned)(ix + iy * g.N);
-    if (idx < NN) {
-      float v[SB];
-      load_slices<SB>(xbb + (int64_t)idx * SB, v);
+  const int64_t t = (int64_t)p * C + c;
+  if (valid) {
+    const RayInfo r = rays[t];
+    traverse_full<kExact>(g, r, ang[p].quadrant, [&](int ix, int iy, float l, float l2) {
+      const unsigned idx = (unsigned)(ix + iy * g.N);
+      if (idx < NN) {
+        float v[SB];
+        load_slices<SB>(xbb + (int64_t)idx * SB, v);
 #pragma unroll
-      for (int k = 0; k < SB; ++k) acc[k] += v[k] * l;
-    }
-    if (MODE == kFwdSIRT) a2 += l2;
-  });
+        for (int k = 0; k < SB; ++k) acc[k] += v[k] * l;
+      }
+      if (MODE == kFwdSIRT) a2 += l2;
+    });
+  }
 #pragma unroll
   for (int k = 0; k < SB; ++k) {
     const int s = b * SB + k;
-    if (s >= S) break;
+    if (s >= S) break;  // uniform across the block
     const int64_t o = ((int64_t)p * S + s) * C + c;
-    const int64_t ow = t * S + s;
-    if (MODE == kFwdY) {
-      out[o] = acc[k];
-    } else if (MODE == kFwdSIRT) {
-      out[ow] = (sino[o] - acc[k]) / a2;
-    } else {
-      out[ow] = sino[o] / acc[k];
+    unsigned m = 0;
+    if (valid) {
+      if (MODE == kFwdY) {
+        out[o] = acc[k];
+      } else {
+        const float wv = (MODE == kFwdSIRT) ? (sino[o] - acc[k]) / a2 : sino[o] / acc[k];
+        out[t * S + s] = wv;
+        m = __float_as_uint(__builtin_fabsf(wv));
+      }
+    }
+    if (MODE != kFwdY) {
+      for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_down((int)m, off, 64));
+      if (lane == 0) atomicMax(maxw + s, m);
     }
   }
 }
 
-// w[p][c][s] = r[p][s][c]
+// w[p][c][s] = r[p][s][c], folding max |r| per slice into maxw (zeroed
+// beforehand).  One block row per (p, s).
 __global__ void __launch_bounds__(256)
-k_to_pcs(const float *__restrict__ r, int P, int S, int C, float *__restrict__ w) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;  // over [P][S][C]
-  if (i >= (int64_t)P * S * C) return;
-  const int c = (int)(i % C);
-  const int64_t ps = i / C;
+k_to_pcs(const float *__restrict__ r, int P, int S, int C, float *__restrict__ w,
+         unsigned *__restrict__ maxw) {
+  const int64_t ps = blockIdx.x;  // p * S + s
   const int s = (int)(ps % S);
   const int64_t p = ps / S;
-  w[(p * C + c) * S + s] = r[i];
+  unsigned m = 0;
+  for (int c = threadIdx.x; c < C; c += 256) {
+    const float v = r[ps * C + c];
+    w[(p * C + c) * S + s] = v;
+    m = max(m, __float_as_uint(__builtin_fabsf(v)));
+  }
+  for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_down((int)m, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(maxw + s, m);
 }
 
 // ------------------------------------------------------- backprojector --
@@ -600,19 +624,21 @@ inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b
 // ------------------------------------------------ launch-shape dispatch --
 
 template <int SB, bool E>
-hipError_t fwd_mode(int mode, const Geometry &g, const RayInfo *rays, const float *xb,
-                    const float *sino, float *out, hipStream_t st) {
-  const int64_t n = g.rays_per_slice();
-  dim3 grid(blocks_for(n, kRayBlock), (g.S + SB - 1) / SB), block(kRayBlock);
+hipError_t fwd_mode(int mode, const Geometry &g, const RayInfo *rays, int ba_log2,
+                    const float *xb, const float *sino, float *out, unsigned *maxw,
+                    hipStream_t st) {
+  const int bc = 64 >> ba_log2, ba = 1 << ba_log2;
+  const int64_t bundles = (int64_t)((g.P + ba - 1) / ba) * ((g.C + bc - 1) / bc);
+  dim3 grid(blocks_for(bundles, kRayBlock / 64), (g.S + SB - 1) / SB), block(kRayBlock);
   if (mode == kFwdY)
     hipLaunchKernelGGL((k_forward<SB, kFwdY, E>), grid, block, 0, st, g.grid, g.d_ang, rays,
-                       g.P, g.S, xb, sino, out);
+                       g.P, g.S, ba_log2, xb, sino, out, maxw);
   else if (mode == kFwdSIRT)
     hipLaunchKernelGGL((k_forward<SB, kFwdSIRT, E>), grid, block, 0, st, g.grid, g.d_ang,
-                       rays, g.P, g.S, xb, sino, out);
+                       rays, g.P, g.S, ba_log2, xb, sino, out, maxw);
   else
     hipLaunchKernelGGL((k_forward<SB, kFwdEM, E>), grid, block, 0, st, g.grid, g.d_ang, rays,
-                       g.P, g.S, xb, sino, out);
+                       g.P, g.S, ba_log2, xb, sino, out, maxw);
   return hipGetLastError();
 }
 
@@ -665,20 +691,24 @@ hipError_t launch_ray_info(const Geometry &g, RayInfo *rays, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_forward(const Geometry &g, const RayInfo *rays, int sb, int mode,
-                          bool exact, const float *xb, const float *sino, float *out,
-                          hipStream_t st) {
+hipError_t launch_forward(const Geometry &g, const RayInfo *rays, int sb, int bundle_angles,
+                          int mode, bool exact, const float *xb, const float *sino,
+                          float *out, unsigned *maxw, hipStream_t st) {
+  int ba_log2 = 0;
+  while ((1 << ba_log2) < bundle_angles && ba_log2 < 6) ++ba_log2;
+  if ((1 << ba_log2) != bundle_angles) return hipErrorInvalidValue;
+#define TRACE_FWD_CASE(SB_)                                                            \
+  case SB_:                                                                            \
+    return exact ? fwd_mode<SB_, true>(mode, g, rays, ba_log2, xb, sino, out, maxw, st) \
+                 : fwd_mode<SB_, false>(mode, g, rays, ba_log2, xb, sino, out, maxw, st);
   switch (sb) {
-    case 4: return exact ? fwd_mode<4, true>(mode, g, rays, xb, sino, out, st)
-                         : fwd_mode<4, false>(mode, g, rays, xb, sino, out, st);
-    case 8: return exact ? fwd_mode<8, true>(mode, g, rays, xb, sino, out, st)
-                         : fwd_mode<8, false>(mode, g, rays, xb, sino, out, st);
-    case 16: return exact ? fwd_mode<16, true>(mode, g, rays, xb, sino, out, st)
-                          : fwd_mode<16, false>(mode, g, rays, xb, sino, out, st);
-    case 32: return exact ? fwd_mode<32, true>(mode, g, rays, xb, sino, out, st)
-                          : fwd_mode<32, false>(mode, g, rays, xb, sino, out, st);
+    TRACE_FWD_CASE(4)
+    TRACE_FWD_CASE(8)
+    TRACE_FWD_CASE(16)
+    TRACE_FWD_CASE(32)
     default: return hipErrorInvalidValue;
   }
+#undef TRACE_FWD_CASE
 }
 
 hipError_t launch_backproject(const Geometry &g, const RayInfo *rays, const int *count,
@@ -811,9 +841,10 @@ hipError_t launch_count_segments(const Geometry &g, const RayInfo *rays,
   return hipGetLastError();
 }
 
-hipError_t launch_to_pcs(const float *r, int P, int S, int C, float *w, hipStream_t st) {
-  const int64_t n = (int64_t)P * S * C;
-  hipLaunchKernelGGL(k_to_pcs, dim3(blocks_for(n, 256)), dim3(256), 0, st, r, P, S, C, w);
+hipError_t launch_to_pcs(const float *r, int P, int S, int C, float *w, unsigned *maxw,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(k_to_pcs, dim3((unsigned)((int64_t)P * S)), dim3(256), 0, st, r, P, S, C,
+                     w, maxw);
   return hipGetLastError();
 }
 

@@ -132,6 +132,11 @@ class Projector:
     def set_launch_shape(self, fwd_batch: int, tile_w: int, tile_h: int, tile_batch: int):
         check(lib().trace_set_launch_shape(self._h, fwd_batch, tile_w, tile_h, tile_batch))
 
+    def set_fwd_bundle(self, angles: int):
+        """Adjacent angles per forward-projector wavefront (1, 2, 4, 8, 16)."""
+        check(lib().trace_set_option(self._h, _capi.TRACE_OPT_FWD_BUNDLE, int(angles)))
+        self._fwd_bundle = int(angles)
+
     def launch_shape(self) -> tuple:
         v = [ct.c_int() for _ in range(4)]
         check(lib().trace_get_launch_shape(self._h, *[ct.byref(x) for x in v]))
