@@ -121,6 +121,8 @@ def main():
     ap.add_argument("--fast-sqrt", action="store_true",
                     help="hardware v_sqrt_f32 segment lengths (default: exact)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default launch shape")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
 
@@ -153,6 +155,24 @@ def main():
     pj = Projector(theta, S, N, center=N / 2.0, device=0, exact_sqrt=not args.fast_sqrt)
     nnz, oor = pj.count_segments()
     log(f"segments per slice: {nnz} (outside image: {oor})")
+    tune = None
+    if not args.no_autotune:
+        # tune on a sub-stack of the same geometry (the shapes depend on the
+        # geometry and the slice batches, not on the stack depth)
+        from trace_amd.autotune import autotune
+        st = min(S, 32)
+        t0 = time.perf_counter()
+        with Projector(theta, st, N, center=N / 2.0, device=0,
+                       exact_sqrt=not args.fast_sqrt) as tp:
+            sub_s = sino[:, :st, :].contiguous()
+            sub_x = torch.full((st, N, N), 0.0 if args.alg == "sirt" else 1.0,
+                               dtype=torch.float32, device=dev)
+            tune = autotune(tp, sub_s, sub_x, args.alg)
+            del sub_s, sub_x
+        pj.set_launch_shape(*tune["shape"])
+        pj.set_fwd_bundle(tune["bundle"])
+        log(f"autotuned shape {tune['shape']} bundle {tune['bundle']} "
+            f"in {time.perf_counter() - t0:.1f} s")
 
     for w in range(args.warmup):
         pj.reconstruct(args.alg, 1, sino, recon)
@@ -243,7 +263,9 @@ def main():
             "step_model_frac": b_alg_step / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS,
         },
         "kernels": per_kernel,
-        "launch_shape": list(pj.launch_shape()),
+        "launch_shape": {"fwd_batch_tile_w_h_batch": list(pj.launch_shape()),
+                         "fwd_bundle_angles": tune["bundle"] if tune else 8,
+                         "autotuned": tune is not None},
         "phases_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in phases.items()},
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,

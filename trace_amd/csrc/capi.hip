@@ -136,8 +136,8 @@ struct trace_handle {
   bool exact = true;
   int backprojector = 0;  // 0 tile (LDS), 1 whole-ray global atomics
   int fwd_sb = 16;        // slices per forward lane (xb layout)
-  int fwd_bundle = 1;     // adjacent angles per forward wavefront
-  TileShape tile{16, 16, 8};
+  int fwd_bundle = 8;     // adjacent angles per forward wavefront
+  TileShape tile{16, 16, 16};
   RayInfo *rays = nullptr;     // [P][C]
   bool rays_ready = false;
   int *count = nullptr;        // [P] columns-per-tile bound for count_shape

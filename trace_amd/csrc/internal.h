@@ -30,7 +30,7 @@ struct Geometry {
 // Launch shape of the tile backprojector: tile of tw x th pixels, sb slices
 // per workgroup (LDS: tw*th*sb floats).
 struct TileShape {
-  int tw = 16, th = 16, sb = 8;
+  int tw = 16, th = 16, sb = 16;
 };
 
 // Regularisation weights for the PML/APMLR stencils, computed on the host

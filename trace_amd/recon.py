@@ -138,6 +138,7 @@ class Projector:
         self._fwd_bundle = int(angles)
 
     def launch_shape(self) -> tuple:
+        """(forward slice batch, tile_w, tile_h, tile slice batch)."""
         v = [ct.c_int() for _ in range(4)]
         check(lib().trace_get_launch_shape(self._h, *[ct.byref(x) for x in v]))
         return tuple(x.value for x in v)
