@@ -203,10 +203,11 @@ struct TilePitch {
 // fraction of the rate of LDS integer atomics on gfx950, so each term len*w
 // is scaled by 2^e (e per slice, such that |len*w| * 2^e < 2^30 for every
 // term: from the slice's largest |w| and the geometry's longest segment),
-// rounded to an integer and added with ds_add_u64.  Integer addition is
+// converted to an integer and added with ds_add_u64.  Integer addition is
 // associative, so the result does not depend on the order in which lanes
 // and waves arrive: the backprojection is bit-reproducible, and its error
-// (at most half a unit of 2^-e per term) is far below that of an fp32 sum.
+// (below one unit of 2^-e per term, i.e. 2^-30 of the largest term) is far
+// below that of an fp32 sum.
 // A slice whose weights are not all finite (y = 0 rays give inf/NaN in the
 // reference too) is accumulated in float instead, in the low word of the
 // same slot, so that inf/NaN propagate as they do in the reference.
@@ -218,9 +219,84 @@ __device__ __forceinline__ int fixed_exponent(float maxw, float maxlen, bool &fi
   return 29 - ilogbf(m);
 }
 
-// Workgroup (tile, slice batch): LDS accumulators acc[SB][TH][TW + 1], all
-// angles; lanes take consecutive (angle, column) pairs of the flattened
-// per-angle column ranges.
+// Main loop of the tile backprojector: lanes take consecutive (angle,
+// column) pairs of the flattened per-angle column ranges and add their
+// segments near the tile into the LDS accumulators.  ALLFX: every slice of
+// the batch uses the fixed-point path (the common case; no per-slice
+// branch in the inner loop).
+template <int TW, int TH, int SB, int MODE, bool kExact, bool ALLFX>
+__device__ __forceinline__ void bp_tile_loop(const TileArgs &a, unsigned long long *acc,
+                                             const int *count, int tx0, int tx1, int ty0,
+                                             int ty1, int b, const int (&ex)[SB],
+                                             const bool (&fx)[SB]) {
+  constexpr int PITCH = TilePitch<TW, TH>::kPitch;
+  constexpr int PLANE = TilePitch<TW, TH>::kPlane;
+  const int N = a.g.N;
+  const int C = a.g.C;
+  const float x0 = grid_line(N, tx0), x1 = grid_line(N, tx1);
+  const float y0 = grid_line(N, ty0), y1 = grid_line(N, ty1);
+  // lane t starts at flat pair t and advances by kTileBlock
+  int p = 0, k = threadIdx.x;
+  while (p < a.P && k >= count[p]) { k -= count[p]; ++p; }
+  while (p < a.P) {
+    int cmin, cmax;
+    const AngleInfo ang = a.ang[p];
+    tile_columns(ang, x0, x1, y0, y1, a.yi0, cmin, cmax);
+    const int c = cmin + k;
+    if (c >= 0 && c < C && c <= cmax) {
+      // weights pre-scaled by 2^e (exact: a power-of-two scaling commutes
+      // with rounding), so fl(len * ws) = fl(len * w) * 2^e per segment
+      float wv[SB];
+      bool any = true;
+      if (MODE == kBpSino) {
+        any = false;
+        const float *wr = a.w + ((int64_t)p * C + c) * a.S + b * SB;
+#pragma unroll
+        for (int q = 0; q < SB; ++q) {
+          const float wq = (b * SB + q < a.S) ? wr[q] : 0.f;
+          any |= (wq != 0.f);
+          wv[q] = (ALLFX || fx[q]) ? ldexpf(wq, ex[q]) : wq;
+        }
+      } else if (MODE == kBpOnes) {
+        wv[0] = ldexpf(1.0f, ex[0]);
+      }
+      if (any) {
+        const RayInfo r = a.rays[(int64_t)p * C + c];
+        traverse_window<kExact>(a.g, r, ang.quadrant, tx0, tx1, ty0, ty1,
+                                [&](int ix, int iy, float l, float) {
+          // ix == N is pixel 0 of the next row (truncation)
+          const bool wrap = (ix == N);
+          ix = wrap ? 0 : ix;
+          iy = wrap ? iy + 1 : iy;
+          const unsigned lx = (unsigned)(ix - tx0);
+          const unsigned ly = (unsigned)(iy - ty0);
+          if (lx < (unsigned)(tx1 - tx0) && ly < (unsigned)(ty1 - ty0)) {
+            unsigned long long *dst = acc + ly * PITCH + lx;
+            if (MODE == kBpCount) {
+              atomicAdd(dst, 1ull);
+            } else {
+#pragma unroll
+              for (int q = 0; q < SB; ++q) {
+                const float t = l * wv[q];
+                if (ALLFX || fx[q]) {
+                  // |t| < 2^30: truncating conversion, error < 2^-e
+                  atomicAdd(dst + q * PLANE, (unsigned long long)(long long)(int)t);
+                } else {
+                  atomicAdd(reinterpret_cast<float *>(dst + q * PLANE), t);
+                }
+              }
+            }
+          }
+        });
+      }
+    }
+    k += kTileBlock;
+    while (p < a.P && k >= count[p]) { k -= count[p]; ++p; }
+  }
+}
+
+// Workgroup (tile, slice batch): LDS accumulators acc[SB][TH][TW + 1] for
+// all angles, then one coalesced write-out.
 template <int TW, int TH, int SB, int MODE, bool kExact>
 __global__ void __launch_bounds__(kTileBlock)
 k_bp_tile(TileArgs a) {
@@ -241,71 +317,20 @@ k_bp_tile(TileArgs a) {
   // per-slice scale exponents (uniform across the workgroup)
   int ex[SB];
   bool fx[SB];
+  bool all_fixed = true;
 #pragma unroll
   for (int q = 0; q < SB; ++q) {
     const int s = b * SB + q;
     float m = 1.0f;
     if (MODE == kBpSino) m = (s < a.S) ? __uint_as_float(a.maxw[s]) : 0.f;
     ex[q] = fixed_exponent(m, a.maxlen, fx[q]);
+    all_fixed &= fx[q];
   }
   __syncthreads();
-
-  const float x0 = grid_line(N, tx0), x1 = grid_line(N, tx1);
-  const float y0 = grid_line(N, ty0), y1 = grid_line(N, ty1);
-  const int C = a.g.C;
-  // lane t starts at flat pair t and advances by kTileBlock
-  int p = 0, k = threadIdx.x;
-  while (p < a.P && k >= count[p]) { k -= count[p]; ++p; }
-  while (p < a.P) {
-    int cmin, cmax;
-    const AngleInfo ang = a.ang[p];
-    tile_columns(ang, x0, x1, y0, y1, a.yi0, cmin, cmax);
-    const int c = cmin + k;
-    if (c >= 0 && c < C && c <= cmax) {
-      float wv[SB];
-      bool any = true;
-      if (MODE == kBpSino) {
-        any = false;
-        const float *wr = a.w + ((int64_t)p * C + c) * a.S + b * SB;
-#pragma unroll
-        for (int q = 0; q < SB; ++q) {
-          wv[q] = (b * SB + q < a.S) ? wr[q] : 0.f;
-          any |= (wv[q] != 0.f);
-        }
-      }
-      if (any) {
-        const RayInfo r = a.rays[(int64_t)p * C + c];
-        traverse_window<kExact>(a.g, r, ang.quadrant, tx0, tx1, ty0, ty1,
-                                [&](int ix, int iy, float l, float) {
-                                  // ix == N is pixel 0 of the next row (truncation)
-                                  const bool wrap = (ix == N);
-                                  ix = wrap ? 0 : ix;
-                                  iy = wrap ? iy + 1 : iy;
-                                  const unsigned lx = (unsigned)(ix - tx0);
-                                  const unsigned ly = (unsigned)(iy - ty0);
-                                  if (lx < (unsigned)(tx1 - tx0) && ly < (unsigned)(ty1 - ty0)) {
-                                    unsigned long long *dst = acc + ly * PITCH + lx;
-                                    if (MODE == kBpCount) {
-                                      atomicAdd(dst, 1ull);
-                                    } else {
-#pragma unroll
-                                      for (int q = 0; q < SB; ++q) {
-                                        const float t = (MODE == kBpSino) ? l * wv[q] : l;
-                                        if (fx[q]) {
-                                          const int ti = (int)rintf(ldexpf(t, ex[q]));
-                                          atomicAdd(dst + q * PLANE, (unsigned long long)(long long)ti);
-                                        } else {
-                                          atomicAdd(reinterpret_cast<float *>(dst + q * PLANE), t);
-                                        }
-                                      }
-                                    }
-                                  }
-                                });
-      }
-    }
-    k += kTileBlock;
-    while (p < a.P && k >= count[p]) { k -= count[p]; ++p; }
-  }
+  if (MODE == kBpCount || all_fixed)
+    bp_tile_loop<TW, TH, SB, MODE, kExact, true>(a, acc, count, tx0, tx1, ty0, ty1, b, ex, fx);
+  else
+    bp_tile_loop<TW, TH, SB, MODE, kExact, false>(a, acc, count, tx0, tx1, ty0, ty1, b, ex, fx);
   __syncthreads();
   // coalesced write-out: plane by plane, row-contiguous
   const int w_ = tx1 - tx0, h_ = ty1 - ty0;
