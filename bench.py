@@ -94,12 +94,13 @@ def cpu_baseline(N, P_full, threads, target_s=12.0):
 
 
 def traffic_record(kernel, workload):
-    """Counter-derived HBM bytes per launch committed under profiles/, or
-    None if no record matches this kernel and workload."""
+    """Counter-derived HBM bytes per launch (FETCH_SIZE and WRITE_SIZE
+    scaled by a copy-kernel calibration) committed under profiles/, or None
+    if no record matches this kernel and workload."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(p) as f:
-            recs = json.load(f)
+            recs = json.load(f)["records"]
     except Exception:
         return None
     for r in recs:
@@ -207,7 +208,8 @@ def main():
         t = ms / max(n, 1) / 1e3
         per_kernel[ph] = {"kernel": kname, "avg_launch_ms": t * 1e3, "launches": n,
                           "algorithmic_bytes_per_launch": nbytes,
-                          "achieved_GBs": nbytes / t / 1e9 if t > 0 else None}
+                          "achieved_GBs": nbytes / t / 1e9 if t > 0 else None,
+                          "traffic_bytes_per_launch": traffic_record(kname, workload)}
     dom = max(("forward", "back"), key=lambda k: per_kernel[k]["avg_launch_ms"])
     kernel = per_kernel[dom]["kernel"]
     t_dom = per_kernel[dom]["avg_launch_ms"] / 1e3
