@@ -120,7 +120,7 @@ void trace_destroy(trace_handle_t *h);
 
 int trace_set_option(trace_handle_t *h, int option, int value);
 
-/* Launch shape: slices per forward-projector lane (4, 8, 16 or 32) and the
+/* Launch shape: slices per forward-projector lane (1, 2, 4, 8, 16 or 32) and the
  * backprojector's image tile (tile_w x tile_h pixels, tile_batch slices per
  * workgroup).  Only the shapes compiled into the library are accepted
  * (TRACE_E_UNSUPPORTED otherwise); trace_amd.autotune picks among them.
@@ -161,6 +161,12 @@ int trace_adjoint(trace_handle_t *h, const float *r, float *z);
  * column sums the reference accumulates as the second replica entry
  * (sirt.cc:62). */
 int trace_column_sums(trace_handle_t *h, float *den);
+
+/* Per-ray sums of one slice's rays into [P][C] device arrays (either may be
+ * null): len_sum = sum of segment lengths (row sums of A) and len2_sum = sum
+ * of squared segment lengths, the SIRT normaliser a2 of UpdateReconReplica
+ * (sirt.cc:47-50), both accumulated in segment order. */
+int trace_ray_sums(trace_handle_t *h, float *len_sum, float *len2_sum);
 
 /* Parity entry: number of segments per pixel of one slice, counts [N][N]
  * (device, float), through the configured backprojector.  Integer-exact, so

@@ -75,6 +75,9 @@ class Oracle:
         L.or_trace_ray.argtypes = [ct.c_int, ct.c_int, ct.c_float, ct.c_float,
                                    ct.c_int, _ip, _fp, _fp]
         L.or_trace_ray.restype = ct.c_int
+        L.or_trace_rays.argtypes = [ct.c_int, ct.c_int, ct.c_float, ct.c_int, _fp,
+                                    _ip, ct.POINTER(ct.c_int64), _ip, _fp, ct.c_int64]
+        L.or_trace_rays.restype = ct.c_int64
         L.or_ray_points.argtypes = [ct.c_int, ct.c_int, ct.c_float, ct.c_float,
                                     ct.c_int, _fp, _fp]
         L.or_ray_points.restype = ct.c_int
@@ -123,6 +126,22 @@ class Oracle:
         n = self.L.or_trace_ray(N, C, center, float(theta), col, _ptr(idx, _ip),
                                 _ptr(ln), _ptr(l2))
         return idx[:n].copy(), ln[:n].copy(), l2[:n].copy()
+
+    def trace_rays(self, N, C, center, theta, col):
+        """Rays (theta[i], col[i]): (offsets [n+1], idx, len) with ray i's
+        segments at [offsets[i], offsets[i+1])."""
+        th = _f32(np.asarray(theta).ravel())
+        cl = np.ascontiguousarray(np.asarray(col).ravel(), dtype=np.int32)
+        n = th.size
+        cap = int(n) * (2 * N + 2)
+        off = np.zeros(n + 1, np.int64)
+        idx = np.zeros(cap, np.int32)
+        ln = np.zeros(cap, np.float32)
+        tot = self.L.or_trace_rays(N, C, center, n, _ptr(th), _ptr(cl, _ip),
+                                   off.ctypes.data_as(ct.POINTER(ct.c_int64)),
+                                   _ptr(idx, _ip), _ptr(ln), cap)
+        assert tot >= 0
+        return off, idx[:tot].copy(), ln[:tot].copy()
 
     def ray_points(self, N, C, center, theta, col):
         px = np.zeros(2 * N + 2, np.float32)

@@ -202,6 +202,34 @@ int or_trace_ray(int N, int C, float center, float theta, int col, int *idx,
   return n;
 }
 
+/* Batch of single rays (theta[i], col[i]): segments concatenated into
+ * idx/len, ray i owning [offsets[i], offsets[i+1]).  Same per-ray code path
+ * as or_trace_ray (trace_utils.cc:48-187 with sirt.cc:116-129's set-up).
+ * Returns the total segment count, or -1 when it would exceed `cap`. */
+int64_t or_trace_rays(int N, int C, float center, int n, const float *theta,
+                      const int *col, int64_t *offsets, int *idx, float *len,
+                      int64_t cap) {
+  or_ws w;
+  if (ws_init(&w, N)) { ws_free(&w); return -1; }
+  int *ti = malloc((2 * (size_t)N + 2) * sizeof(int));
+  float *tl = malloc((2 * (size_t)N + 2) * sizeof(float));
+  const float mov = or_mov(C, center);
+  int64_t total = 0;
+  offsets[0] = 0;
+  for (int r = 0; r < n; ++r) {
+    const float q = theta[r];
+    int k = trace_one(&w, or_yi(C, col[r], mov), sinf(q), cosf(q), or_quadrant(q),
+                      ti, tl, NULL);
+    if (k < 0 || total + k > cap) { total = -1; break; }
+    memcpy(idx + total, ti, (size_t)k * sizeof(int));
+    memcpy(len + total, tl, (size_t)k * sizeof(float));
+    total += k;
+    offsets[r + 1] = total;
+  }
+  free(ti); free(tl); ws_free(&w);
+  return total;
+}
+
 /* Merged intersection points of one ray (debugging aid; same code path). */
 int or_ray_points(int N, int C, float center, float theta, int col, float *px,
                   float *py) {

@@ -13,7 +13,8 @@ from conftest import adversarial_angles, rel_l2
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(16, 16, 16, 8), (4, 16, 16, 16), (8, 16, 16, 4), (32, 24, 24, 8), (4, 32, 16, 8),
-          (4, 16, 32, 8), (16, 32, 32, 4), (8, 32, 32, 8), (4, 8, 8, 16), (4, 64, 64, 2)]
+          (4, 16, 32, 8), (16, 32, 32, 4), (8, 32, 32, 8), (4, 8, 8, 16), (4, 64, 64, 2),
+          (1, 32, 32, 1), (2, 64, 64, 1), (1, 32, 32, 2), (2, 16, 16, 2)]
 
 
 def _oracle_counts(oracle, th, N, center):
@@ -81,7 +82,7 @@ def test_adjoint_vs_oracle(gpu, oracle, shape):
     assert rel_l2(za, z64) < 1e-6
 
 
-@pytest.mark.parametrize("fb", [4, 8, 16, 32])
+@pytest.mark.parametrize("fb", [1, 2, 4, 8, 16, 32])
 def test_forward_batches_bitexact(gpu, oracle, fb):
     torch = gpu
     from trace_amd import Projector

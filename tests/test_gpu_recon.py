@@ -135,7 +135,8 @@ def test_nonfinite_sinogram_rejected(gpu):
 
 
 @pytest.mark.parametrize("shape", [(4, 64, 64, 2), (8, 32, 32, 8), (32, 16, 16, 16),
-                                   (16, 16, 16, 8), (16, 24, 24, 8)])
+                                   (16, 16, 16, 8), (16, 24, 24, 8), (1, 32, 32, 1),
+                                   (2, 64, 64, 1), (2, 16, 16, 2)])
 @pytest.mark.parametrize("alg", ["sirt", "mlem"])
 def test_launch_shapes_recon(gpu, oracle, alg, shape):
     """Every compiled launch shape gives the same reconstruction."""
@@ -154,3 +155,28 @@ def test_launch_shapes_recon(gpu, oracle, alg, shape):
         x = dx.cpu().numpy()
     xo = oracle.reconstruct(alg, th, sino, init, N / 2, iters)
     assert rel_l2(x, xo) < TOL, shape
+
+
+@pytest.mark.parametrize("fb", [1, 2, 4, 8])
+@pytest.mark.parametrize("alg,compat", [("pml", 1), ("apmlr", 1), ("apmlr", 0)])
+def test_regularised_slice_batches(gpu, oracle, alg, compat, fb):
+    """The fused stencil update refreshes the slice-batched image copy for
+    every forward slice batch; 8 slices fill the last batch exactly, so the
+    compat APMLR slot of the last slice lies past the batched copy."""
+    import torch
+    from trace_amd import Projector, RegParams
+    from trace_amd.phantom import angles_rad, noisy_sinogram
+    N, P, S, iters = 40, 30, 8, 4
+    th = angles_rad(P)
+    sino = noisy_sinogram(th, N, S, seed=11, I0=1e4, K=6).numpy()
+    init = np.ones((S, N, N), np.float32)
+    prm = Params(beta=0.1, beta1=1.0, delta=0.01, delta1=1.0, regw=1.0, apmlr_compat=compat)
+    rp = RegParams(0.1, 1.0, 0.01, 1.0, 1.0, bool(compat))
+    print(f"{alg} compat={compat} forward batch {fb}", flush=True)
+    with Projector(th, S, N, N / 2) as pj:
+        pj.set_launch_shape(fb, 16, 16, 4)
+        dx = torch.from_numpy(init).cuda()
+        pj.reconstruct(alg, iters, torch.from_numpy(sino).cuda(), dx, rp)
+        x = dx.cpu().numpy()
+    xo = oracle.reconstruct(alg, th, sino, init, N / 2, iters, prm)
+    assert rel_l2(x, xo) < TOL

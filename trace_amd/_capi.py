@@ -99,6 +99,7 @@ _SIGS = {
     "trace_forward": (ct.c_int, [_vp, _vp, _vp]),
     "trace_adjoint": (ct.c_int, [_vp, _vp, _vp]),
     "trace_column_sums": (ct.c_int, [_vp, _vp]),
+    "trace_ray_sums": (ct.c_int, [_vp, _vp, _vp]),
     "trace_dump_rays": (ct.c_int, [_vp, ct.c_int, _vp, _vp, ct.c_int, _vp, _vp,
                                    _vp, _vp]),
     "trace_count_segments": (ct.c_int, [_vp, ct.POINTER(ct.c_int64),

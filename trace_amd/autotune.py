@@ -2,7 +2,7 @@
 backprojector.
 
 The forward projector's shape is its slice batch (slices gathered per
-segment, 4/8/16); the backprojector's is its image tile and slice batch
+segment, 1 to 32) and ray-bundle width; the backprojector's is its image tile and slice batch
 (tile_w x tile_h pixels x tile_batch slices of LDS accumulators).  The two
 kernels are independent, so each is tuned on its own: every candidate runs
 one iteration of the real workload and the phase timer (HIP events on the
@@ -14,10 +14,27 @@ from __future__ import annotations
 import json
 import os
 
-FWD_BATCHES = (4, 8, 16, 32)
+FWD_BATCHES = (1, 2, 4, 8, 16, 32)
 BUNDLES = (1, 2, 4, 8, 16)
 TILES = ((16, 16, 8), (16, 16, 16), (16, 16, 4), (24, 24, 8), (32, 16, 8),
-         (16, 32, 8), (32, 32, 4), (32, 32, 8), (8, 8, 16), (64, 64, 2))
+         (16, 32, 8), (32, 32, 4), (32, 32, 8), (8, 8, 16), (64, 64, 2),
+         (32, 32, 1), (64, 64, 1), (32, 32, 2), (16, 16, 2))
+
+
+def _pow2_ceil(n: int) -> int:
+    p = 1
+    while p < n:
+        p *= 2
+    return p
+
+
+def batch_candidates(batches, S: int):
+    """Slice batches worth timing for a stack of S slices: no more than one
+    batch of padding (<= max(4, S rounded up to a power of two)) and no more
+    than 8 launches' worth of repeated traversal (>= S / 8)."""
+    hi = max(4, _pow2_ceil(S))
+    c = [b for b in batches if b <= hi and 8 * b >= S]
+    return c or [max(batches)]
 
 _CACHE: dict = {}
 
@@ -49,12 +66,11 @@ def autotune(pj, sino, recon, alg: str = "mlem", params=None,
         return res
     work = recon.clone()
     fwd_ms, back_ms = {}, {}
-    base_tile = tiles[0]
+    tile_batches = batch_candidates(sorted({t[2] for t in tiles}), pj.S)
+    base_tile = next(t for t in tiles if t[2] in tile_batches)
     pj.set_phase_timing(True)
     try:
-        for fb in fwd_batches:
-            if fb > max(4, pj.S):
-                continue  # all but one batch would be padding
+        for fb in batch_candidates(fwd_batches, pj.S):
             for ba in bundles:
                 pj.set_launch_shape(fb, *base_tile)
                 pj.set_fwd_bundle(ba)
@@ -70,6 +86,8 @@ def autotune(pj, sino, recon, alg: str = "mlem", params=None,
         best_fb, best_ba = min(fwd_ms, key=fwd_ms.get)
         pj.set_fwd_bundle(best_ba)
         for t in tiles:
+            if t[2] not in tile_batches:
+                continue
             pj.set_launch_shape(best_fb, *t)
             work.copy_(recon)
             pj.reconstruct(alg, 1, sino, work, params)

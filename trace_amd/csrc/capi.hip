@@ -63,11 +63,10 @@ hipError_t dev_alloc(T **p, size_t count) {
 
 // Launch shapes compiled into the library (kernels.hip, launch_backproject).
 bool tile_shape_supported(int tw, int th, int sb) {
-  static const int shapes[][3] = {{16, 16, 8}, {16, 16, 16}, {16, 16, 4}, {24, 24, 8},
-                                  {32, 16, 8}, {16, 32, 8}, {32, 32, 4}, {32, 32, 8},
-                                  {8, 8, 16},  {64, 64, 2}};
-  for (auto &s : shapes)
-    if (s[0] == tw && s[1] == th && s[2] == sb) return true;
+#define TRACE_SHAPE_MATCH(TW_, TH_, SB_) \
+  if (tw == TW_ && th == TH_ && sb == SB_) return true;
+  TRACE_BP_SHAPES(TRACE_SHAPE_MATCH)
+#undef TRACE_SHAPE_MATCH
   return false;
 }
 
@@ -450,8 +449,9 @@ int trace_set_launch_shape(trace_handle_t *h, int fwd_batch, int tile_w, int til
                            int tile_batch) {
   g_err.clear();
   if (!h) return fail(TRACE_E_INVALID, "handle is null");
-  if (fwd_batch != 4 && fwd_batch != 8 && fwd_batch != 16 && fwd_batch != 32)
-    return fail(TRACE_E_UNSUPPORTED, "forward slice batch must be 4, 8, 16 or 32");
+  if (fwd_batch != 1 && fwd_batch != 2 && fwd_batch != 4 && fwd_batch != 8 &&
+      fwd_batch != 16 && fwd_batch != 32)
+    return fail(TRACE_E_UNSUPPORTED, "forward slice batch must be 1, 2, 4, 8, 16 or 32");
   if (!tile_shape_supported(tile_w, tile_h, tile_batch))
     return fail(TRACE_E_UNSUPPORTED, "tile shape not compiled into this library");
   h->fwd_sb = fwd_batch;
@@ -510,8 +510,7 @@ int trace_reconstruct(trace_handle_t *h, trace_alg_t alg, int iters,
       if (int rc = forward(h, fmode, sino, h->w)) return rc;
       if (int rc = backproject(h, kBpOutSino, h->w, h->z, TRACE_PHASE_BACK)) return rc;
       TIMED(TRACE_PHASE_UPDATE, launch_update_pml(cur, alt, h->z, h->den, S, g.N, prm->beta,
-                                                  st, nullptr, nullptr));
-      TIMED(TRACE_PHASE_UPDATE, launch_pack(alt, S, 0, 0.f, NN, nslots, sb, h->xb, st));
+                                                  sb, nslots, h->xb, st));
       std::swap(cur, alt);
     }
     if (cur != recon)
@@ -534,8 +533,7 @@ int trace_reconstruct(trace_handle_t *h, trace_alg_t alg, int iters,
     if (int rc = forward(h, kFwdOutEM, sino, h->w)) return rc;
     if (int rc = backproject(h, kBpOutSino, h->w, h->z, TRACE_PHASE_BACK)) return rc;
     TIMED(TRACE_PHASE_UPDATE, launch_update_apmlr(pa, pb, h->z, h->den, S, g.N, rp, w,
-                                                  prm->apmlr_compat, st, nullptr, nullptr));
-    TIMED(TRACE_PHASE_UPDATE, launch_pack(pb, S + 2, off, 0.f, NN, nslots, sb, h->xb, st));
+                                                  prm->apmlr_compat, sb, nslots, h->xb, st));
     std::swap(pa, pb);
   }
   HIP_TRY(hipMemcpyAsync(recon, pa + NN, sizeof(float) * S * NN, hipMemcpyDeviceToDevice, st));
@@ -616,6 +614,17 @@ int trace_segment_counts(trace_handle_t *h, float *counts) {
   if (!counts) return fail(TRACE_E_INVALID, "null buffer");
   HIP_TRY(hipSetDevice(h->device));
   return backproject(h, kBpOutCount, nullptr, counts, TRACE_PHASE_SETUP);
+}
+
+int trace_ray_sums(trace_handle_t *h, float *len_sum, float *len2_sum) {
+  g_err.clear();
+  if (!h) return fail(TRACE_E_INVALID, "handle is null");
+  if (!len_sum && !len2_sum) return TRACE_OK;
+  HIP_TRY(hipSetDevice(h->device));
+  if (int rc = ensure_rays(h)) return rc;
+  TIMED(TRACE_PHASE_SETUP,
+        launch_ray_sums(h->geo, h->rays, h->exact, len_sum, len2_sum, h->stream));
+  return TRACE_OK;
 }
 
 int trace_dump_rays(trace_handle_t *h, int n, const int *proj, const int *col,

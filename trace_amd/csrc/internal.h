@@ -56,6 +56,13 @@ hipError_t launch_ray_info(const Geometry &g, RayInfo *rays, hipStream_t st);
 hipError_t launch_forward(const Geometry &g, const RayInfo *rays, int sb, int bundle_angles,
                           int mode, bool exact, const float *xb, const float *sino,
                           float *out, unsigned *maxw, hipStream_t st);
+// Tile shapes (tile_w, tile_h, slices per workgroup) compiled into the
+// backprojector; the C-ABI accepts exactly these.
+#define TRACE_BP_SHAPES(X)                                                 \
+  X(16, 16, 8) X(16, 16, 16) X(16, 16, 4) X(24, 24, 8) X(32, 16, 8)          \
+  X(16, 32, 8) X(32, 32, 4) X(32, 32, 8) X(8, 8, 16) X(64, 64, 2)           \
+  X(32, 32, 1) X(64, 64, 1) X(32, 32, 2) X(16, 16, 2)
+
 hipError_t launch_backproject(const Geometry &g, const RayInfo *rays, const int *count,
                               const TileShape &ts, int mode, bool exact, const float *w,
                               const unsigned *maxw, float maxlen, float *z, hipStream_t st);
@@ -69,17 +76,19 @@ hipError_t launch_update_em(bool sirt, float *x, const float *z, const float *de
                             int64_t NN, int sb, float *xb, hipStream_t st);
 hipError_t launch_fill(float *p, int64_t n, float v, hipStream_t st);
 hipError_t launch_update_pml(const float *x, float *xn, const float *z, const float *den,
-                             int S, int N, float beta, hipStream_t st, float *F_out,
-                             float *G_out);
+                             int S, int N, float beta, int sb, int nslots, float *xb,
+                             hipStream_t st);
 hipError_t launch_update_apmlr(const float *xpad, float *xnpad, const float *z,
                                const float *den, int S, int N, const RegParams &p,
-                               const StencilWeights &w, int compat, hipStream_t st,
-                               float *F_out, float *G_out);
+                               const StencilWeights &w, int compat, int sb, int nslots,
+                               float *xb, hipStream_t st);
 hipError_t launch_adjoint_atomic(const Geometry &g, const RayInfo *rays, int mode, bool exact,
                                  const float *r, float *z, int nslices, hipStream_t st);
 hipError_t launch_dump_rays(const Geometry &g, const RayInfo *rays, int n, const int *proj,
                             const int *col, int max_seg, int *counts, int *idx, float *len,
                             float *len2, hipStream_t st, bool exact);
+hipError_t launch_ray_sums(const Geometry &g, const RayInfo *rays, bool exact, float *len_sum,
+                           float *len2_sum, hipStream_t st);
 hipError_t launch_count_segments(const Geometry &g, const RayInfo *rays,
                                  unsigned long long *total, unsigned long long *oor,
                                  unsigned *maxlen, hipStream_t st);

@@ -63,11 +63,31 @@ struct Vec {
 
 template <int SB>
 __device__ __forceinline__ void load_slices(const float *__restrict__ p, float (&o)[SB]) {
-  static_assert(SB % 4 == 0, "slice batch must be a multiple of 4");
+  static_assert(SB == 1 || SB == 2 || SB % 4 == 0, "slice batch: 1, 2 or a multiple of 4");
+  if constexpr (SB == 1) {
+    o[0] = p[0];
+  } else if constexpr (SB == 2) {
+    const float2 q = *reinterpret_cast<const float2 *>(p);
+    o[0] = q.x; o[1] = q.y;
+  } else {
 #pragma unroll
-  for (int k = 0; k < SB; k += 4) {
-    const float4 q = *reinterpret_cast<const float4 *>(p + k);
-    o[k] = q.x; o[k + 1] = q.y; o[k + 2] = q.z; o[k + 3] = q.w;
+    for (int k = 0; k < SB; k += 4) {
+      const float4 q = *reinterpret_cast<const float4 *>(p + k);
+      o[k] = q.x; o[k + 1] = q.y; o[k + 2] = q.z; o[k + 3] = q.w;
+    }
+  }
+}
+
+template <int SB>
+__device__ __forceinline__ void store_slices(float *__restrict__ p, const float (&v)[SB]) {
+  if constexpr (SB == 1) {
+    p[0] = v[0];
+  } else if constexpr (SB == 2) {
+    *reinterpret_cast<float2 *>(p) = make_float2(v[0], v[1]);
+  } else {
+#pragma unroll
+    for (int q = 0; q < SB; q += 4)
+      *reinterpret_cast<float4 *>(p + q) = make_float4(v[q], v[q + 1], v[q + 2], v[q + 3]);
   }
 }
 
@@ -387,8 +407,7 @@ k_pack(const float *__restrict__ src, int nsrc, int off, float fill, int64_t NN,
     v[q] = (t < nslots && s >= 0 && s < nsrc) ? src[(int64_t)s * NN + j] : fill;
   }
   float *dst = xb + ((int64_t)b * NN + j) * SB;
-#pragma unroll
-  for (int q = 0; q < SB; q += 4) *reinterpret_cast<float4 *>(dst + q) = make_float4(v[q], v[q + 1], v[q + 2], v[q + 3]);
+  store_slices<SB>(dst, v);
 }
 
 // SIRT (sirt.cc:21-33) / MLEM (mlem.cc:21-34) update of a batch of slices
@@ -416,8 +435,7 @@ k_update_em(float *__restrict__ x, const float *__restrict__ z, const float *__r
     }
   }
   float *dst = xb + ((int64_t)b * NN + j) * SB;
-#pragma unroll
-  for (int q = 0; q < SB; q += 4) *reinterpret_cast<float4 *>(dst + q) = make_float4(v[q], v[q + 1], v[q + 2], v[q + 3]);
+  store_slices<SB>(dst, v);
 }
 
 __global__ void k_fill(float *__restrict__ p, int64_t n, float v) {
@@ -469,89 +487,125 @@ __device__ __forceinline__ int pixel_class(int m, int n, int N) {
 // Fused PML stencil (pml.cc:25-201, F and G from zero) and closed-form
 // update (pml.cc:3-23):  x' = (-G + sqrt(G^2 - 8 num F)) / (4F), with
 // num = -x * (A^T w) (the x factor of pml.cc:243 taken out of the sum).
+// A thread owns pixel (m, n) of the SB slices of slot batch b and also
+// writes the slice-batched copy xb of the new image for the next ray pass
+// (slot t = slice t; padding slots hold 0).
+template <int SB>
 __global__ void __launch_bounds__(kPixBlock)
 k_update_pml(const float *__restrict__ x, float *__restrict__ xn,
              const float *__restrict__ z, const float *__restrict__ den,
-             int S, int N, float beta, float *__restrict__ F_out,
-             float *__restrict__ G_out) {
+             int S, int N, float beta, float *__restrict__ xb) {
   const int m = blockIdx.x * kPixBlock + threadIdx.x;
   const int n = blockIdx.y;
-  const int s = blockIdx.z;
+  const int b = blockIdx.z;
   if (m >= N) return;
   const int64_t NN = (int64_t)N * N;
-  const int64_t j = (int64_t)s * NN + (int64_t)n * N + m;
+  const int64_t pix = (int64_t)n * N + m;
   const int cls = pixel_class(m, n, N);
   const float *w = c_pml_w[cls == 0 ? 0 : (cls <= 4 ? 1 : 2)];
-  const float x0 = x[j];
-  float f = 0.f, gg = 0.f;
   const int cnt = c_nbn[cls];
-  for (int q = 0; q < cnt; ++q) {
-    const float xq = x[j + c_nb[cls][q][0] + (int64_t)c_nb[cls][q][1] * N];
-    f += 2.0f * beta * w[q];
-    gg -= 2.0f * beta * w[q] * (x0 + xq);
+  const float d = den[pix];
+  float v[SB];
+#pragma unroll
+  for (int q = 0; q < SB; ++q) {
+    const int s = b * SB + q;
+    v[q] = 0.f;
+    if (s < S) {
+      const int64_t j = (int64_t)s * NN + pix;
+      const float x0 = x[j];
+      float f = 0.f, gg = 0.f;
+      for (int k = 0; k < cnt; ++k) {
+        const float xq = x[j + c_nb[cls][k][0] + (int64_t)c_nb[cls][k][1] * N];
+        f += 2.0f * beta * w[k];
+        gg -= 2.0f * beta * w[k] * (x0 + xq);
+      }
+      const float G = gg + d;
+      const float F = f;
+      const float nm = -(x0 * z[j]);
+      const float xv = (-G + __builtin_sqrtf(G * G - 8.0f * nm * F)) / (4.0f * F);
+      xn[j] = xv;
+      v[q] = xv;
+    }
   }
-  const float G = gg + den[(int64_t)n * N + m];
-  const float F = f;
-  const float nm = -(x0 * z[j]);
-  if (F_out) { F_out[j] = F; G_out[j] = G; }
-  xn[j] = (-G + __builtin_sqrtf(G * G - 8.0f * nm * F)) / (4.0f * F);
+  store_slices<SB>(xb + ((int64_t)b * NN + pix) * SB, v);
 }
 
 struct ApmlrArgs {
   StencilWeights w;
   RegParams p;
   int compat;
+  int nslots;  // slots of the batched copy
 };
 
 // Fused APMLR stencil (apmlr.cc:919-982 with FGInner/FGTop/FGBottom) and
 // update (apmlr.cc:3-26) over ghost-padded images [S+2][N][N]; real slice s
 // lives at padded index s+1.  The ray pass of slice s used padded slice s
 // (compat, apmlr.cc:1082-1084) or s+1, which also supplies num's x factor.
+// Thread (pixel, slot batch b): slot t of the batched copy holds padded
+// slice t + off (off = 0 compat, 1 otherwise); real slices are updated,
+// ghosts (padded 0 and S+1, constant) are copied, slots past S+1 hold 0.
+// The grid covers every real slice even when its slot lies past nslots.
+template <int SB>
 __global__ void __launch_bounds__(kPixBlock)
 k_update_apmlr(const float *__restrict__ xp, float *__restrict__ xnp,
                const float *__restrict__ z, const float *__restrict__ den,
-               int S, int N, ApmlrArgs a, float *__restrict__ F_out,
-               float *__restrict__ G_out) {
+               int S, int N, ApmlrArgs a, float *__restrict__ xb) {
   const int m = blockIdx.x * kPixBlock + threadIdx.x;
   const int n = blockIdx.y;
-  const int s = blockIdx.z;
+  const int b = blockIdx.z;
   if (m >= N) return;
   const int64_t NN = (int64_t)N * N;
-  const int64_t jp = (int64_t)(s + 1) * NN + (int64_t)n * N + m;  // padded
-  const int64_t j = (int64_t)s * NN + (int64_t)n * N + m;
-  const bool top = (s == 0), bottom = (s == S - 1 && S - 1 != 0);
-  const int zmode = top ? 1 : (bottom ? 2 : 0);
-  const int nz = zmode == 0 ? 2 : 1;
+  const int64_t pix = (int64_t)n * N + m;
   const int cls = pixel_class(m, n, N);
-  const float *w = a.w.w[zmode][cls];
   const int cnt = c_nbn[cls];
-  const float x0 = xp[jp];
-  float f = 0.f, gg = 0.f;
+  const int off = a.compat ? 0 : 1;
   const float tb = 2.0f * a.p.beta;
-  for (int q = 0; q < cnt; ++q) {
-    const float xq = xp[jp + c_nb[cls][q][0] + (int64_t)c_nb[cls][q][1] * N];
-    const float mg = x0 + xq;
-    const float rg = x0 - xq;
-    const float gm = 1.0f / (1.0f + __builtin_fabsf(rg / a.p.delta));
-    f += tb * w[q] * gm;
-    gg -= tb * w[q] * gm * mg;
-  }
   const float tb1 = 2.0f * a.p.beta1;
-  for (int q = 0; q < nz; ++q) {
-    const int64_t dz = (zmode == 2 || q == 1) ? -NN : NN;
-    const float xq = xp[jp + dz];
-    const float mg = x0 + xq;
-    const float rg = x0 - xq;
-    const float gm = 1.0f / (1.0f + __builtin_fabsf(rg / a.p.delta1));
-    f += tb1 * w[cnt + q] * gm;
-    gg -= tb1 * w[cnt + q] * gm * mg;
+  const float d = den[pix];
+  float v[SB];
+#pragma unroll
+  for (int q = 0; q < SB; ++q) {
+    const int pi = b * SB + q + off;  // padded slice of slot b*SB+q
+    float val = 0.f;
+    if (pi >= 1 && pi <= S) {
+      const int s = pi - 1;
+      const int64_t jp = (int64_t)pi * NN + pix;
+      const int64_t j = (int64_t)s * NN + pix;
+      const bool top = (s == 0), bottom = (s == S - 1 && S - 1 != 0);
+      const int zmode = top ? 1 : (bottom ? 2 : 0);
+      const int nz = zmode == 0 ? 2 : 1;
+      const float *w = a.w.w[zmode][cls];
+      const float x0 = xp[jp];
+      float f = 0.f, gg = 0.f;
+      for (int k = 0; k < cnt; ++k) {
+        const float xq = xp[jp + c_nb[cls][k][0] + (int64_t)c_nb[cls][k][1] * N];
+        const float mg = x0 + xq;
+        const float rg = x0 - xq;
+        const float gm = 1.0f / (1.0f + __builtin_fabsf(rg / a.p.delta));
+        f += tb * w[k] * gm;
+        gg -= tb * w[k] * gm * mg;
+      }
+      for (int k = 0; k < nz; ++k) {
+        const int64_t dz = (zmode == 2 || k == 1) ? -NN : NN;
+        const float xq = xp[jp + dz];
+        const float mg = x0 + xq;
+        const float rg = x0 - xq;
+        const float gm = 1.0f / (1.0f + __builtin_fabsf(rg / a.p.delta1));
+        f += tb1 * w[cnt + k] * gm;
+        gg -= tb1 * w[cnt + k] * gm * mg;
+      }
+      const float G = gg + d;
+      const float F = f;
+      const float xpass = a.compat ? xp[jp - NN] : x0;
+      const float nm = -(xpass * z[j]);
+      val = (-G + __builtin_sqrtf(G * G - 8.0f * nm * F)) / (4.0f * F);
+      xnp[jp] = val;
+    } else if (pi == 0 || pi == S + 1) {
+      val = xp[(int64_t)pi * NN + pix];
+    }
+    v[q] = val;
   }
-  const float G = gg + den[(int64_t)n * N + m];
-  const float F = f;
-  const float xpass = a.compat ? xp[jp - NN] : x0;
-  const float nm = -(xpass * z[j]);
-  if (F_out) { F_out[j] = F; G_out[j] = G; }
-  xnp[jp] = (-G + __builtin_sqrtf(G * G - 8.0f * nm * F)) / (4.0f * F);
+  if (b * SB < a.nslots) store_slices<SB>(xb + ((int64_t)b * NN + pix) * SB, v);
 }
 
 // ------------------------------------------------- reference / debugging --
@@ -566,7 +620,7 @@ k_adjoint_atomic(GridInfo g, const AngleInfo *__restrict__ ang,
   const int64_t t = (int64_t)blockIdx.x * kRayBlock + threadIdx.x;
   const int C = g.C;
   if (t >= (int64_t)P * C) return;
-  const int c = (int)(t % C), p = (int)(t / C);
+  const int p = (int)(t / C);
   const int s = blockIdx.y;
   const unsigned NN = (unsigned)g.N * (unsigned)g.N;
   float w = 1.f;
@@ -602,6 +656,26 @@ __global__ void k_dump_rays(GridInfo g, const AngleInfo *__restrict__ ang,
                                           ++k;
                                         });
   counts[t] = cnt;
+}
+
+// Per-ray sums over all segments, in the walk order: len_sum = sum len
+// (row sums of A) and len2_sum = sum len^2 (the SIRT normaliser a2 of
+// sirt.cc:47-50, accumulated from the squared lengths as there).  Output
+// [P][C]; either pointer may be null.
+template <bool kExact>
+__global__ void __launch_bounds__(kRayBlock)
+k_ray_sums(GridInfo g, const AngleInfo *__restrict__ ang, const RayInfo *__restrict__ rays,
+           int P, float *__restrict__ len_sum, float *__restrict__ len2_sum) {
+  const int64_t t = (int64_t)blockIdx.x * kRayBlock + threadIdx.x;
+  if (t >= (int64_t)P * g.C) return;
+  const int p = (int)(t / g.C);
+  float a1 = 0.f, a2 = 0.f;
+  traverse_full<kExact>(g, rays[t], ang[p].quadrant, [&](int, int, float l, float l2) {
+    a1 += l;
+    a2 += l2;
+  });
+  if (len_sum) len_sum[t] = a1;
+  if (len2_sum) len2_sum[t] = a2;
 }
 
 // Segment statistics of one slice: total count, count outside the image,
@@ -727,6 +801,8 @@ hipError_t launch_forward(const Geometry &g, const RayInfo *rays, int sb, int bu
     return exact ? fwd_mode<SB_, true>(mode, g, rays, ba_log2, xb, sino, out, maxw, st) \
                  : fwd_mode<SB_, false>(mode, g, rays, ba_log2, xb, sino, out, maxw, st);
   switch (sb) {
+    TRACE_FWD_CASE(1)
+    TRACE_FWD_CASE(2)
     TRACE_FWD_CASE(4)
     TRACE_FWD_CASE(8)
     TRACE_FWD_CASE(16)
@@ -755,16 +831,7 @@ hipError_t launch_backproject(const Geometry &g, const RayInfo *rays, const int 
 #define TRACE_BP_CASE(TW_, TH_, SB_)                              \
   if (ts.tw == TW_ && ts.th == TH_ && ts.sb == SB_)               \
     return bp_mode<TW_, TH_, SB_>(mode, exact, a, nb, st);
-  TRACE_BP_CASE(16, 16, 8)
-  TRACE_BP_CASE(16, 16, 16)
-  TRACE_BP_CASE(16, 16, 4)
-  TRACE_BP_CASE(24, 24, 8)
-  TRACE_BP_CASE(32, 16, 8)
-  TRACE_BP_CASE(16, 32, 8)
-  TRACE_BP_CASE(32, 32, 4)
-  TRACE_BP_CASE(32, 32, 8)
-  TRACE_BP_CASE(8, 8, 16)
-  TRACE_BP_CASE(64, 64, 2)
+  TRACE_BP_SHAPES(TRACE_BP_CASE)
 #undef TRACE_BP_CASE
   return hipErrorInvalidValue;
 }
@@ -774,6 +841,8 @@ hipError_t launch_pack(const float *src, int nsrc, int off, float fill, int64_t 
   const int nb = (nslots + sb - 1) / sb;
   dim3 grid(blocks_for(NN, kPixBlock), nb), block(kPixBlock);
   switch (sb) {
+    case 1: hipLaunchKernelGGL(k_pack<1>, grid, block, 0, st, src, nsrc, off, fill, NN, nslots, xb); break;
+    case 2: hipLaunchKernelGGL(k_pack<2>, grid, block, 0, st, src, nsrc, off, fill, NN, nslots, xb); break;
     case 4: hipLaunchKernelGGL(k_pack<4>, grid, block, 0, st, src, nsrc, off, fill, NN, nslots, xb); break;
     case 8: hipLaunchKernelGGL(k_pack<8>, grid, block, 0, st, src, nsrc, off, fill, NN, nslots, xb); break;
     case 16: hipLaunchKernelGGL(k_pack<16>, grid, block, 0, st, src, nsrc, off, fill, NN, nslots, xb); break;
@@ -795,6 +864,8 @@ hipError_t launch_update_em(bool sirt, float *x, const float *z, const float *de
       hipLaunchKernelGGL((k_update_em<SB_, false>), grid, block, 0, st, x, z, den, S, NN, xb); \
     break;
   switch (sb) {
+    TRACE_UPD_CASE(1)
+    TRACE_UPD_CASE(2)
     TRACE_UPD_CASE(4)
     TRACE_UPD_CASE(8)
     TRACE_UPD_CASE(16)
@@ -812,22 +883,51 @@ hipError_t launch_fill(float *p, int64_t n, float v, hipStream_t st) {
 }
 
 hipError_t launch_update_pml(const float *x, float *xn, const float *z, const float *den,
-                             int S, int N, float beta, hipStream_t st, float *F_out,
-                             float *G_out) {
-  dim3 grid(blocks_for(N, kPixBlock), N, S);
-  hipLaunchKernelGGL(k_update_pml, grid, dim3(kPixBlock), 0, st, x, xn, z, den, S, N, beta,
-                     F_out, G_out);
+                             int S, int N, float beta, int sb, int nslots, float *xb,
+                             hipStream_t st) {
+  dim3 grid(blocks_for(N, kPixBlock), N, nslots / sb);
+#define TRACE_PML_CASE(SB_)                                                              \
+  case SB_:                                                                              \
+    hipLaunchKernelGGL(k_update_pml<SB_>, grid, dim3(kPixBlock), 0, st, x, xn, z, den, S, \
+                       N, beta, xb);                                                      \
+    break;
+  switch (sb) {
+    TRACE_PML_CASE(1)
+    TRACE_PML_CASE(2)
+    TRACE_PML_CASE(4)
+    TRACE_PML_CASE(8)
+    TRACE_PML_CASE(16)
+    TRACE_PML_CASE(32)
+    default: return hipErrorInvalidValue;
+  }
+#undef TRACE_PML_CASE
   return hipGetLastError();
 }
 
 hipError_t launch_update_apmlr(const float *xpad, float *xnpad, const float *z,
                                const float *den, int S, int N, const RegParams &p,
-                               const StencilWeights &w, int compat, hipStream_t st,
-                               float *F_out, float *G_out) {
-  dim3 grid(blocks_for(N, kPixBlock), N, S);
-  ApmlrArgs a{w, p, compat};
-  hipLaunchKernelGGL(k_update_apmlr, grid, dim3(kPixBlock), 0, st, xpad, xnpad, z, den, S,
-                     N, a, F_out, G_out);
+                               const StencilWeights &w, int compat, int sb, int nslots,
+                               float *xb, hipStream_t st) {
+  // batches: every slot of the copy and every real slice's slot
+  const int off = compat ? 0 : 1;
+  const int need = (nslots > S + 1 - off) ? nslots : S + 1 - off;
+  dim3 grid(blocks_for(N, kPixBlock), N, (need + sb - 1) / sb);
+  ApmlrArgs a{w, p, compat, nslots};
+#define TRACE_APMLR_CASE(SB_)                                                         \
+  case SB_:                                                                           \
+    hipLaunchKernelGGL(k_update_apmlr<SB_>, grid, dim3(kPixBlock), 0, st, xpad, xnpad, z, \
+                       den, S, N, a, xb);                                              \
+    break;
+  switch (sb) {
+    TRACE_APMLR_CASE(1)
+    TRACE_APMLR_CASE(2)
+    TRACE_APMLR_CASE(4)
+    TRACE_APMLR_CASE(8)
+    TRACE_APMLR_CASE(16)
+    TRACE_APMLR_CASE(32)
+    default: return hipErrorInvalidValue;
+  }
+#undef TRACE_APMLR_CASE
   return hipGetLastError();
 }
 
@@ -854,6 +954,18 @@ hipError_t launch_dump_rays(const Geometry &g, const RayInfo *rays, int n, const
   else
     hipLaunchKernelGGL(k_dump_rays<false>, dim3(blocks_for(n, 64)), dim3(64), 0, st, g.grid,
                        g.d_ang, rays, n, proj, col, max_seg, counts, idx, len, len2);
+  return hipGetLastError();
+}
+
+hipError_t launch_ray_sums(const Geometry &g, const RayInfo *rays, bool exact, float *len_sum,
+                           float *len2_sum, hipStream_t st) {
+  const int64_t n = g.rays_per_slice();
+  if (exact)
+    hipLaunchKernelGGL(k_ray_sums<true>, dim3(blocks_for(n, kRayBlock)), dim3(kRayBlock), 0, st,
+                       g.grid, g.d_ang, rays, g.P, len_sum, len2_sum);
+  else
+    hipLaunchKernelGGL(k_ray_sums<false>, dim3(blocks_for(n, kRayBlock)), dim3(kRayBlock), 0, st,
+                       g.grid, g.d_ang, rays, g.P, len_sum, len2_sum);
   return hipGetLastError();
 }
 

@@ -183,6 +183,13 @@ class Projector:
         check(lib().trace_column_sums(self._h, _dptr(d)))
         return d
 
+    def ray_sums(self):
+        """(sum len, sum len^2) per ray of one slice, each [P][C]."""
+        a1 = self._empty(self.P, self.C)
+        a2 = self._empty(self.P, self.C)
+        check(lib().trace_ray_sums(self._h, _dptr(a1), _dptr(a2)))
+        return a1, a2
+
     def reconstruct(self, alg: str, iters: int, sino, recon,
                     params: RegParams | None = None):
         """Runs `iters` iterations in place on device tensors
