@@ -195,6 +195,61 @@ int trace_phase_times(trace_handle_t *h, double *ms, int64_t *launches, int rese
 void trace_deg2rad(const float *deg, float *rad, int n);
 void trace_absolute(float *v, int64_t n);
 
+/* ---- Batched reconstruction of slices with different geometries ------
+ *
+ * The reference reconstructs the slices of one dataset (one geometry) per
+ * run (sirt_main.cc:145-202, mlem_main.cc:139-182).  A batch takes many
+ * slices, each with its own detector width C (image C x C), angle set,
+ * centre, algorithm (SIRT or MLEM, with the preprocessing and initial
+ * image of its main already applied) and iteration count, and runs them
+ * together: one HIP-graph launch per trace_batch_run, whose persistent
+ * kernels take (slice, work item) pairs from device work queues.  Each
+ * slice's result is the one trace_reconstruct gives for it alone.
+ * Buffers stay the caller's; their addresses are fixed at creation. */
+typedef struct trace_batch trace_batch_t;
+
+typedef struct {
+  int num_projs;        /* P */
+  int num_cols;         /* C; the image is C x C (N = C, sirt.cc:182) */
+  float center;         /* rotation axis; <= 0 -> C/2 + 1 (trace_data.h:97) */
+  const float *theta;   /* [P] radians (host; copied at creation) */
+  trace_alg_t alg;      /* TRACE_SIRT or TRACE_MLEM */
+  int iters;            /* iterations of this slice */
+  const float *sino;    /* device [P][C] */
+  float *recon;         /* device [C][C]: initial image in, result out */
+} trace_batch_slice_t;
+
+/* Builds the per-slice tables (synchronous) on `device`, ordered on
+ * `hip_stream`. */
+int trace_batch_create(trace_batch_t **b, const trace_batch_slice_t *slices, int n,
+                       int device, void *hip_stream);
+
+/* Ray-bundle width (adjacent angles per forward wavefront: 1, 2, 4, 8, 16)
+ * and backprojector tile (16x16, 32x32 or 64x64); trace_amd.autotune picks
+ * them.  Default 8 and 32x32. */
+int trace_batch_set_launch_shape(trace_batch_t *b, int bundle_angles, int tile_w, int tile_h);
+int trace_batch_get_launch_shape(trace_batch_t *b, int *bundle_angles, int *tile_w,
+                                 int *tile_h);
+
+/* TRACE_OPT_EXACT_SQRT as for a handle. */
+int trace_batch_set_option(trace_batch_t *b, int option, int value);
+
+/* Captures and instantiates the graph trace_batch_run(b, max_iters) will
+ * launch, without launching it (graphs are cached per iteration count and
+ * dropped when the launch shape or options change). */
+int trace_batch_prepare(trace_batch_t *b, int max_iters);
+
+/* Runs every slice's iterations (at most max_iters each when max_iters > 0)
+ * as one graph launch, asynchronous on the batch's stream after checking
+ * that no sinogram holds NaN/Inf (TRACE_E_NONFINITE, synchronous). */
+int trace_batch_run(trace_batch_t *b, int max_iters);
+
+/* Create, run, synchronise, destroy. */
+int trace_batch_reconstruct(const trace_batch_slice_t *slices, int n, int device,
+                            void *hip_stream);
+
+void trace_batch_destroy(trace_batch_t *b);
+
 /* Thread-local text of the last error ("" if none). */
 const char *trace_last_error(void);
 

@@ -77,6 +77,34 @@ def test_null_handle_calls():
     L.trace_destroy(None)  # accepted
 
 
+def test_batch_argument_validation():
+    """Batch creation rejects bad slices before any HIP call."""
+    L = _capi.lib()
+    from trace_amd._capi import TraceBatchSlice
+    h = ct.c_void_p()
+    th = np.zeros(4, np.float32)
+    arr = (TraceBatchSlice * 2)()
+    for a in arr:
+        a.num_projs, a.num_cols, a.center, a.iters = 4, 8, 4.0, 3
+        a.theta = th.ctypes.data_as(ct.POINTER(ct.c_float))
+        a.alg = _capi.TRACE_SIRT
+        a.sino, a.recon = 16, 32  # never dereferenced on these paths
+    assert L.trace_batch_create(None, arr, 2, 0, None) == _capi.TRACE_E_INVALID
+    assert L.trace_batch_create(ct.byref(h), arr, 0, 0, None) == _capi.TRACE_E_INVALID
+    arr[1].alg = _capi.TRACE_PML
+    assert L.trace_batch_create(ct.byref(h), arr, 2, 0, None) == _capi.TRACE_E_UNSUPPORTED
+    assert "SIRT or MLEM" in L.trace_last_error().decode()
+    arr[1].alg = _capi.TRACE_MLEM
+    arr[0].iters = -1
+    assert L.trace_batch_create(ct.byref(h), arr, 2, 0, None) == _capi.TRACE_E_INVALID
+    arr[0].iters = 3
+    arr[0].sino = None
+    assert L.trace_batch_create(ct.byref(h), arr, 2, 0, None) == _capi.TRACE_E_INVALID
+    assert L.trace_batch_run(None, 0) == _capi.TRACE_E_INVALID
+    assert L.trace_batch_set_launch_shape(None, 8, 32, 32) == _capi.TRACE_E_INVALID
+    L.trace_batch_destroy(None)  # accepted
+
+
 def test_python_wrapper_raises():
     from trace_amd import Projector, TraceInvalid
     with pytest.raises(TraceInvalid):

@@ -52,6 +52,19 @@ class TraceParams(ct.Structure):
     ]
 
 
+class TraceBatchSlice(ct.Structure):
+    _fields_ = [
+        ("num_projs", ct.c_int),
+        ("num_cols", ct.c_int),
+        ("center", ct.c_float),
+        ("theta", ct.POINTER(ct.c_float)),
+        ("alg", ct.c_int),
+        ("iters", ct.c_int),
+        ("sino", ct.c_void_p),
+        ("recon", ct.c_void_p),
+    ]
+
+
 class TraceError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"trace error {code}: {msg}")
@@ -108,6 +121,16 @@ _SIGS = {
                                      ct.POINTER(ct.c_int64), ct.c_int]),
     "trace_deg2rad": (None, [_vp, _vp, ct.c_int]),
     "trace_absolute": (None, [_vp, ct.c_int64]),
+    "trace_batch_create": (ct.c_int, [ct.POINTER(_vp), ct.POINTER(TraceBatchSlice), ct.c_int,
+                                       ct.c_int, _vp]),
+    "trace_batch_set_launch_shape": (ct.c_int, [_vp, ct.c_int, ct.c_int, ct.c_int]),
+    "trace_batch_get_launch_shape": (ct.c_int, [_vp, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int),
+                                                ct.POINTER(ct.c_int)]),
+    "trace_batch_set_option": (ct.c_int, [_vp, ct.c_int, ct.c_int]),
+    "trace_batch_run": (ct.c_int, [_vp, ct.c_int]),
+    "trace_batch_prepare": (ct.c_int, [_vp, ct.c_int]),
+    "trace_batch_reconstruct": (ct.c_int, [ct.POINTER(TraceBatchSlice), ct.c_int, ct.c_int, _vp]),
+    "trace_batch_destroy": (None, [_vp]),
     "trace_last_error": (ct.c_char_p, []),
     "trace_version": (ct.c_char_p, []),
 }

@@ -18,155 +18,27 @@
 #include <vector>
 
 #include "../../include/trace_hip.h"
+#include "handle.h"
 #include "internal.h"
 
 using namespace trace_hip;
 
-namespace {
 
+namespace trace_hip {
+
+namespace {
 thread_local std::string g_err;
+}  // namespace
 
 int fail(int code, const std::string &msg) {
   g_err = msg;
   return code;
 }
 
-#define HIP_TRY(expr)                                                        \
-  do {                                                                       \
-    hipError_t e_ = (expr);                                                  \
-    if (e_ != hipSuccess) {                                                  \
-      return fail(e_ == hipErrorOutOfMemory ? TRACE_E_NOMEM : TRACE_E_HIP,   \
-                  std::string(#expr) + ": " + hipGetErrorString(e_));        \
-    }                                                                        \
-  } while (0)
+int nbatch(int S, int sb) { return (S + sb - 1) / sb; }
 
-constexpr float kPI = 3.14159265358979f;  // trace_utils.h:8
-
-// trace_utils.cc:41-46
-int quadrant_of(float q) {
-  return ((q >= 0 && q < kPI / 2) || (q >= kPI && q < 3 * kPI / 2)) ? 1 : 0;
-}
-
-// trace_data.h:97-101: default centre, mov and its unconditional nudge.
-float mov_of(int C, float center) {
-  float c = center;
-  if (c <= 0.) c = (float)((double)(float)C / 2. + 1.);
-  float mov = (float)((double)(float)C / 2. - (double)c);
-  if ((double)(mov - std::ceil(mov)) < 1e-6) mov = (float)((double)mov + 1e-6);
-  return mov;
-}
-
-template <typename T>
-hipError_t dev_alloc(T **p, size_t count) {
-  return hipMalloc((void **)p, count * sizeof(T));
-}
-
-// Launch shapes compiled into the library (kernels.hip, launch_backproject).
-bool tile_shape_supported(int tw, int th, int sb) {
-#define TRACE_SHAPE_MATCH(TW_, TH_, SB_) \
-  if (tw == TW_ && th == TH_ && sb == SB_) return true;
-  TRACE_BP_SHAPES(TRACE_SHAPE_MATCH)
-#undef TRACE_SHAPE_MATCH
-  return false;
-}
-
-// HIP-event brackets around kernel launches, per phase.
-struct PhaseTimer {
-  bool on = false;
-  std::vector<hipEvent_t> pool;
-  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
-  double ms[TRACE_NUM_PHASES] = {0, 0, 0, 0};
-  int64_t count[TRACE_NUM_PHASES] = {0, 0, 0, 0};
-  hipEvent_t open_ev = nullptr;
-  int open_phase = -1;
-
-  hipEvent_t take() {
-    if (!pool.empty()) {
-      hipEvent_t e = pool.back();
-      pool.pop_back();
-      return e;
-    }
-    hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
-    return e;
-  }
-  void begin(int phase, hipStream_t st) {
-    if (!on) return;
-    open_ev = take();
-    open_phase = phase;
-    if (open_ev) (void)hipEventRecord(open_ev, st);
-  }
-  void end(hipStream_t st) {
-    if (!on || !open_ev) return;
-    hipEvent_t e = take();
-    if (!e) return;
-    (void)hipEventRecord(e, st);
-    pending.push_back({open_phase, {open_ev, e}});
-    open_ev = nullptr;
-  }
-  void collect() {
-    for (auto &p : pending) {
-      float t = 0.f;
-      if (hipEventSynchronize(p.second.second) == hipSuccess &&
-          hipEventElapsedTime(&t, p.second.first, p.second.second) == hipSuccess) {
-        ms[p.first] += t;
-        count[p.first] += 1;
-      }
-      pool.push_back(p.second.first);
-      pool.push_back(p.second.second);
-    }
-    pending.clear();
-  }
-  void release() {
-    collect();
-    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
-    pool.clear();
-  }
-};
-
-}  // namespace
-
-struct trace_handle {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  Geometry geo;
-  std::vector<float> theta;
-  std::vector<AngleInfo> ang;
-  bool exact = true;
-  int backprojector = 0;  // 0 tile (LDS), 1 whole-ray global atomics
-  int fwd_sb = 16;        // slices per forward lane (xb layout)
-  int fwd_bundle = 8;     // adjacent angles per forward wavefront
-  TileShape tile{16, 16, 16};
-  RayInfo *rays = nullptr;     // [P][C]
-  bool rays_ready = false;
-  int *count = nullptr;        // [P] columns-per-tile bound for count_shape
-  TileShape count_shape{0, 0, 0};
-  float *den = nullptr;        // [N][N]
-  bool den_ready = false;
-  bool den_exact = true;
-  float *xb = nullptr;         // [ceil(S/fwd_sb)][N][N][fwd_sb]
-  int xb_sb = 0;
-  float *w = nullptr;          // [P][C][S] per-ray weights
-  float *z = nullptr;          // [S][N][N]
-  float *xa = nullptr;         // [S+2][N][N] work image (PML/APMLR)
-  float *xc = nullptr;         // [S+2][N][N]
-  int *flag = nullptr;
-  unsigned long long *counters = nullptr;  // [3]: segments, outside image, max length bits
-  unsigned *maxw = nullptr;    // [S] bits of max |w| per slice
-  float maxlen = 0.f;          // longest segment of the geometry
-  int64_t nnz = 0, nnz_oor = 0;
-  PhaseTimer timer;
-};
-
-// Launch bracketed by the phase timer.
-#define TIMED(phase, expr)              \
-  do {                                  \
-    h->timer.begin((phase), h->stream); \
-    HIP_TRY(expr);                      \
-    h->timer.end(h->stream);            \
-  } while (0)
-
-namespace {
+const char *last_error() { return g_err.c_str(); }
+void clear_error() { g_err.clear(); }
 
 // Ray table and segment statistics (count, longest segment) of the
 // geometry; one synchronous read-back per handle.
@@ -229,27 +101,14 @@ int ensure_den(trace_handle *h) {
   return TRACE_OK;
 }
 
-int nbatch(int S, int sb) { return (S + sb - 1) / sb; }
-
-// Forward projection of xb into weights w (mode kFwdOutSIRT/EM) or y
-// (kFwdOutY), with max |w| per slice folded into h->maxw.
-int forward(trace_handle *h, int mode, const float *sino, float *out) {
-  if (mode != kFwdOutY)
-    TIMED(TRACE_PHASE_SETUP,
-          launch_fill(reinterpret_cast<float *>(h->maxw), h->geo.S, 0.f, h->stream));
-  TIMED(TRACE_PHASE_FORWARD, launch_forward(h->geo, h->rays, h->fwd_sb, h->fwd_bundle, mode,
-                                            h->exact, h->xb, sino, out, h->maxw, h->stream));
-  return TRACE_OK;
-}
-
-int ensure_work(trace_handle *h, bool images) {
+int ensure_work(trace_handle *h, bool images, bool batched_copy) {
   const size_t nn = (size_t)h->geo.NN();
   const size_t S = (size_t)h->geo.S;
   if (h->xb && h->xb_sb != h->fwd_sb) {
     (void)hipFree(h->xb);
     h->xb = nullptr;
   }
-  if (!h->xb) {
+  if (!h->xb && batched_copy) {
     HIP_TRY(dev_alloc(&h->xb, (size_t)nbatch((int)S, h->fwd_sb) * h->fwd_sb * nn));
     h->xb_sb = h->fwd_sb;
   }
@@ -260,6 +119,57 @@ int ensure_work(trace_handle *h, bool images) {
     if (!h->xa) HIP_TRY(dev_alloc(&h->xa, (S + 2) * nn));
     if (!h->xc) HIP_TRY(dev_alloc(&h->xc, (S + 2) * nn));
   }
+  return TRACE_OK;
+}
+
+int check_finite(trace_handle *h, const float *v, int64_t n) {
+  HIP_TRY(hipMemsetAsync(h->flag, 0, sizeof(int), h->stream));
+  HIP_TRY(launch_check_finite(v, n, h->flag, h->stream));
+  int flag = 0;
+  HIP_TRY(hipMemcpyAsync(&flag, h->flag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (flag) return fail(TRACE_E_NONFINITE, "sinogram holds NaN or Inf values");
+  return TRACE_OK;
+}
+
+}  // namespace trace_hip
+
+namespace {
+
+constexpr float kPI = 3.14159265358979f;  // trace_utils.h:8
+
+// trace_utils.cc:41-46
+int quadrant_of(float q) {
+  return ((q >= 0 && q < kPI / 2) || (q >= kPI && q < 3 * kPI / 2)) ? 1 : 0;
+}
+
+// trace_data.h:97-101: default centre, mov and its unconditional nudge.
+float mov_of(int C, float center) {
+  float c = center;
+  if (c <= 0.) c = (float)((double)(float)C / 2. + 1.);
+  float mov = (float)((double)(float)C / 2. - (double)c);
+  if ((double)(mov - std::ceil(mov)) < 1e-6) mov = (float)((double)mov + 1e-6);
+  return mov;
+}
+
+// Launch shapes compiled into the library (kernels.hip, launch_backproject).
+bool tile_shape_supported(int tw, int th, int sb) {
+#define TRACE_SHAPE_MATCH(TW_, TH_, SB_) \
+  if (tw == TW_ && th == TH_ && sb == SB_) return true;
+  TRACE_BP_SHAPES(TRACE_SHAPE_MATCH)
+#undef TRACE_SHAPE_MATCH
+  return false;
+}
+
+
+// Forward projection of xb into weights w (mode kFwdOutSIRT/EM) or y
+// (kFwdOutY), with max |w| per slice folded into h->maxw.
+int forward(trace_handle *h, int mode, const float *sino, float *out) {
+  if (mode != kFwdOutY)
+    TIMED(TRACE_PHASE_SETUP,
+          launch_fill(reinterpret_cast<float *>(h->maxw), h->geo.S, 0.f, h->stream));
+  TIMED(TRACE_PHASE_FORWARD, launch_forward(h->geo, h->rays, h->fwd_sb, h->fwd_bundle, mode,
+                                            h->exact, h->xb, sino, out, h->maxw, h->stream));
   return TRACE_OK;
 }
 
@@ -282,16 +192,6 @@ void apmlr_weights(float regw, StencilWeights &w) {
       for (int q = 0; q < nz; ++q) w.w[zm][cls][nin[cls] + q] = regw / total;
     }
   }
-}
-
-int check_finite(trace_handle *h, const float *v, int64_t n) {
-  HIP_TRY(hipMemsetAsync(h->flag, 0, sizeof(int), h->stream));
-  HIP_TRY(launch_check_finite(v, n, h->flag, h->stream));
-  int flag = 0;
-  HIP_TRY(hipMemcpyAsync(&flag, h->flag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
-  HIP_TRY(hipStreamSynchronize(h->stream));
-  if (flag) return fail(TRACE_E_NONFINITE, "sinogram holds NaN or Inf values");
-  return TRACE_OK;
 }
 
 int validate_params(trace_alg_t alg, const trace_params_t *prm) {

@@ -56,12 +56,47 @@ hipError_t launch_ray_info(const Geometry &g, RayInfo *rays, hipStream_t st);
 hipError_t launch_forward(const Geometry &g, const RayInfo *rays, int sb, int bundle_angles,
                           int mode, bool exact, const float *xb, const float *sino,
                           float *out, unsigned *maxw, hipStream_t st);
+// One slice of a batched reconstruction: its own geometry (P, C = N,
+// centre), tables and buffers; one slice per geometry.
+struct BatchSliceDev {
+  GridInfo g;
+  const AngleInfo *ang;   // [P]
+  const RayInfo *rays;    // [P][C]
+  const int *count;       // [P] columns-per-tile bound for the batch tile
+  int P;
+  int alg;                // 0 SIRT, 1 MLEM
+  int iters;
+  float yi0;
+  float maxlen;           // longest segment of the geometry
+  const float *sino;      // [P][C]
+  float *x;               // [N][N] image (updated in place)
+  float *w;               // [P][C] per-ray weights
+  float *z;               // [N][N] A^T w
+  const float *den;       // [N][N] A^T 1
+  unsigned *maxw;         // bits of max |w| of this slice
+};
+
+// Work queues of one batch: phase f's items of slice order_f[k] are
+// [start_f[k], start_f[k+1]); workgroups of a persistent kernel take items
+// from counters[f] until they run out.  Slices are ordered by decreasing
+// cost per item, so the tail of each phase is filled with short items.
+struct BatchQueues {
+  const int *order[3];    // forward, backprojector, update
+  const int *start[3];    // [n + 1] each
+  int n;
+  unsigned *counters;     // [3]
+  unsigned *maxw_all;     // [n]
+};
+
+// Compiled tile shapes of the batched backprojector (one slice per tile).
+#define TRACE_BATCH_TILES(X) X(16, 16) X(32, 32) X(64, 64)
+
 // Tile shapes (tile_w, tile_h, slices per workgroup) compiled into the
 // backprojector; the C-ABI accepts exactly these.
 #define TRACE_BP_SHAPES(X)                                                 \
   X(16, 16, 8) X(16, 16, 16) X(16, 16, 4) X(24, 24, 8) X(32, 16, 8)          \
   X(16, 32, 8) X(32, 32, 4) X(32, 32, 8) X(8, 8, 16) X(64, 64, 2)           \
-  X(32, 32, 1) X(64, 64, 1) X(32, 32, 2) X(16, 16, 2)
+  X(32, 32, 1) X(64, 64, 1) X(32, 32, 2) X(16, 16, 2) X(16, 16, 1)
 
 hipError_t launch_backproject(const Geometry &g, const RayInfo *rays, const int *count,
                               const TileShape &ts, int mode, bool exact, const float *w,
@@ -82,6 +117,17 @@ hipError_t launch_update_apmlr(const float *xpad, float *xnpad, const float *z,
                                const float *den, int S, int N, const RegParams &p,
                                const StencilWeights &w, int compat, int sb, int nslots,
                                float *xb, hipStream_t st);
+hipError_t launch_batch_reset(const BatchQueues &q, hipStream_t st);
+hipError_t launch_batch_forward(const BatchSliceDev *d, const BatchQueues &q, int it,
+                                int bundle_angles, bool exact, int grid, hipStream_t st);
+hipError_t launch_batch_backproject(const BatchSliceDev *d, const BatchQueues &q, int it,
+                                    int tw, int th, bool exact, int grid, size_t lds,
+                                    hipStream_t st);
+hipError_t launch_batch_update(const BatchSliceDev *d, const BatchQueues &q, int it, int grid,
+                               hipStream_t st);
+// Resident workgroups per CU of the batched kernels (0 if not compiled).
+int batch_blocks_per_cu(int phase, int tw, int th, bool exact, size_t lds);
+bool batch_tile_supported(int tw, int th);
 hipError_t launch_adjoint_atomic(const Geometry &g, const RayInfo *rays, int mode, bool exact,
                                  const float *r, float *z, int nslices, hipStream_t st);
 hipError_t launch_dump_rays(const Geometry &g, const RayInfo *rays, int n, const int *proj,

@@ -101,20 +101,20 @@ __device__ __forceinline__ void store_slices(float *__restrict__ p, const float 
 // to w[p][c][s] (slices innermost, read as a block by the backprojector) and
 // fold max |w| per slice into maxw (bits; zeroed beforehand).
 template <int SB, int MODE, bool kExact>
-__global__ void __launch_bounds__(kRayBlock)
-k_forward(GridInfo g, const AngleInfo *__restrict__ ang, const RayInfo *__restrict__ rays,
-          int P, int S, int ba_log2, const float *__restrict__ xb,
-          const float *__restrict__ sino, float *__restrict__ out,
-          unsigned *__restrict__ maxw) {
+__device__ __forceinline__ void forward_bundle(const GridInfo &g, const AngleInfo *__restrict__ ang,
+                                               const RayInfo *__restrict__ rays, int P, int S,
+                                               int ba_log2, const float *__restrict__ xb,
+                                               const float *__restrict__ sino,
+                                               float *__restrict__ out,
+                                               unsigned *__restrict__ maxw, int64_t bundle,
+                                               int b) {
   const int C = g.C;
   const int lane = threadIdx.x & 63;
   const int bc_log2 = 6 - ba_log2;  // columns per bundle = 64 / BA
   const int nbc = (C + (1 << bc_log2) - 1) >> bc_log2;
-  const int64_t bundle = (int64_t)blockIdx.x * (kRayBlock / 64) + (threadIdx.x >> 6);
   const int p = (int)(bundle / nbc) * (1 << ba_log2) + (lane >> bc_log2);
   const int c = (int)(bundle % nbc) * (1 << bc_log2) + (lane & ((1 << bc_log2) - 1));
   const bool valid = (p < P) && (c < C);
-  const int b = blockIdx.y;
   const unsigned NN = (unsigned)g.N * (unsigned)g.N;
   const float *xbb = xb + (int64_t)b * NN * SB;
   float acc[SB];
@@ -155,6 +155,17 @@ k_forward(GridInfo g, const AngleInfo *__restrict__ ang, const RayInfo *__restri
       if (lane == 0) atomicMax(maxw + s, m);
     }
   }
+}
+
+template <int SB, int MODE, bool kExact>
+__global__ void __launch_bounds__(kRayBlock)
+k_forward(GridInfo g, const AngleInfo *__restrict__ ang, const RayInfo *__restrict__ rays,
+          int P, int S, int ba_log2, const float *__restrict__ xb,
+          const float *__restrict__ sino, float *__restrict__ out,
+          unsigned *__restrict__ maxw) {
+  forward_bundle<SB, MODE, kExact>(g, ang, rays, P, S, ba_log2, xb, sino, out, maxw,
+                                   (int64_t)blockIdx.x * (kRayBlock / 64) + (threadIdx.x >> 6),
+                                   blockIdx.y);
 }
 
 // w[p][c][s] = r[p][s][c], folding max |r| per slice into maxw (zeroed
@@ -316,21 +327,23 @@ __device__ __forceinline__ void bp_tile_loop(const TileArgs &a, unsigned long lo
 }
 
 // Workgroup (tile, slice batch): LDS accumulators acc[SB][TH][TW + 1] for
-// all angles, then one coalesced write-out.
+// all angles, then one coalesced write-out.  `acc` is the workgroup's
+// dynamic LDS ([SB][TH][TW + 1] 64-bit words, then int count[P] when
+// a.count_lds); the body starts and ends with a workgroup barrier, so a
+// persistent workgroup may run it for one tile after another.
 template <int TW, int TH, int SB, int MODE, bool kExact>
-__global__ void __launch_bounds__(kTileBlock)
-k_bp_tile(TileArgs a) {
-  extern __shared__ unsigned long long acc[];  // [SB][TH][TW + 1], then int count[P]
+__device__ __forceinline__ void bp_tile_body(const TileArgs &a, int tile, int b,
+                                             unsigned long long *acc) {
   constexpr int PITCH = TilePitch<TW, TH>::kPitch;
   constexpr int PLANE = TilePitch<TW, TH>::kPlane;
   int *lcount = reinterpret_cast<int *>(acc + SB * PLANE);
   const int *count = a.count_lds ? lcount : a.count;
   const int N = a.g.N;
   const int tiles_x = (N + TW - 1) / TW;
-  const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+  const int tx = tile % tiles_x, ty = tile / tiles_x;
   const int tx0 = tx * TW, ty0 = ty * TH;
   const int tx1 = min(tx0 + TW, N), ty1 = min(ty0 + TH, N);
-  const int b = blockIdx.y;
+  __syncthreads();  // the previous tile's write-out has read acc
   for (int i = threadIdx.x; i < SB * PLANE; i += kTileBlock) acc[i] = 0ull;
   if (a.count_lds)
     for (int i = threadIdx.x; i < a.P; i += kTileBlock) lcount[i] = a.count[i];
@@ -372,6 +385,13 @@ k_bp_tile(TileArgs a) {
       zs[(int64_t)(ty0 + ly) * N + tx0 + lx] = out;
     }
   }
+}
+
+template <int TW, int TH, int SB, int MODE, bool kExact>
+__global__ void __launch_bounds__(kTileBlock)
+k_bp_tile(TileArgs a) {
+  extern __shared__ unsigned long long acc[];
+  bp_tile_body<TW, TH, SB, MODE, kExact>(a, blockIdx.x, blockIdx.y, acc);
 }
 
 // Per-slice maximum of |w| over w [P][C][S] (float bits, atomicMax on the
@@ -606,6 +626,111 @@ k_update_apmlr(const float *__restrict__ xp, float *__restrict__ xnp,
     v[q] = val;
   }
   if (b * SB < a.nslots) store_slices<SB>(xb + ((int64_t)b * NN + pix) * SB, v);
+}
+
+// ------------------------------------------------------------- batched --
+//
+// Persistent kernels over the work queues of a batch of slices with
+// different geometries (BatchQueues).  Each workgroup takes an item, finds
+// its slice by binary search in the phase's start table and runs the same
+// per-item code as the single-geometry kernels (one slice per geometry).
+
+__device__ __forceinline__ int batch_next_item(unsigned *counter) {
+  __shared__ unsigned s_item;
+  __syncthreads();  // every lane has read the previous item
+  if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
+  __syncthreads();
+  return __builtin_amdgcn_readfirstlane((int)s_item);
+}
+
+__device__ __forceinline__ int batch_find(const int *__restrict__ start, int n, int item) {
+  int lo = 0, hi = n;  // start[lo] <= item < start[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (start[mid] <= item) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void k_batch_reset(unsigned *__restrict__ counters, unsigned *__restrict__ maxw,
+                              int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 3) counters[i] = 0u;
+  if (i < n) maxw[i] = 0u;
+}
+
+template <bool kExact>
+__global__ void __launch_bounds__(kRayBlock)
+k_forward_batch(const BatchSliceDev *__restrict__ d, BatchQueues q, int it, int ba_log2) {
+  const int *start = q.start[0];
+  const int total = start[q.n];
+  for (;;) {
+    const int item = batch_next_item(q.counters + 0);
+    if (item >= total) break;
+    const int k = batch_find(start, q.n, item);
+    const BatchSliceDev &sd = d[__builtin_amdgcn_readfirstlane(q.order[0][k])];
+    if (it >= sd.iters) continue;
+    const int64_t bundle = (int64_t)(item - start[k]) * (kRayBlock / 64) + (threadIdx.x >> 6);
+    if (sd.alg == 0)
+      forward_bundle<1, kFwdSIRT, kExact>(sd.g, sd.ang, sd.rays, sd.P, 1, ba_log2, sd.x,
+                                          sd.sino, sd.w, sd.maxw, bundle, 0);
+    else
+      forward_bundle<1, kFwdEM, kExact>(sd.g, sd.ang, sd.rays, sd.P, 1, ba_log2, sd.x, sd.sino,
+                                        sd.w, sd.maxw, bundle, 0);
+  }
+}
+
+template <int TW, int TH, bool kExact>
+__global__ void __launch_bounds__(kTileBlock)
+k_bp_batch(const BatchSliceDev *__restrict__ d, BatchQueues q, int it) {
+  extern __shared__ unsigned long long acc[];  // [TH][TW + 1], then int count[P]
+  const int *start = q.start[1];
+  const int total = start[q.n];
+  for (;;) {
+    const int item = batch_next_item(q.counters + 1);
+    if (item >= total) break;
+    const int k = batch_find(start, q.n, item);
+    const BatchSliceDev &sd = d[__builtin_amdgcn_readfirstlane(q.order[1][k])];
+    if (it >= sd.iters) continue;
+    TileArgs a;
+    a.g = sd.g;
+    a.ang = sd.ang;
+    a.rays = sd.rays;
+    a.count = sd.count;
+    a.P = sd.P;
+    a.S = 1;
+    a.count_lds = 1;
+    a.yi0 = sd.yi0;
+    a.w = sd.w;
+    a.maxw = sd.maxw;
+    a.maxlen = sd.maxlen;
+    a.z = sd.z;
+    bp_tile_body<TW, TH, 1, kBpSino, kExact>(a, item - start[k], 0, acc);
+  }
+}
+
+// SIRT (sirt.cc:21-33) / MLEM (mlem.cc:21-34) update, kPixBlock pixels per
+// item.
+__global__ void __launch_bounds__(kPixBlock)
+k_update_batch(const BatchSliceDev *__restrict__ d, BatchQueues q, int it) {
+  const int *start = q.start[2];
+  const int total = start[q.n];
+  for (;;) {
+    const int item = batch_next_item(q.counters + 2);
+    if (item >= total) break;
+    const int k = batch_find(start, q.n, item);
+    const BatchSliceDev &sd = d[__builtin_amdgcn_readfirstlane(q.order[2][k])];
+    if (it >= sd.iters) continue;
+    const int64_t NN = (int64_t)sd.g.N * sd.g.N;
+    const int64_t j = (int64_t)(item - start[k]) * kPixBlock + threadIdx.x;
+    if (j < NN) {
+      float xv = sd.x[j];
+      if (sd.alg == 0) xv += sd.z[j] / sd.den[j];
+      else xv *= sd.z[j] / sd.den[j];
+      sd.x[j] = xv;
+    }
+  }
 }
 
 // ------------------------------------------------- reference / debugging --
@@ -929,6 +1054,84 @@ hipError_t launch_update_apmlr(const float *xpad, float *xnpad, const float *z,
   }
 #undef TRACE_APMLR_CASE
   return hipGetLastError();
+}
+
+hipError_t launch_batch_reset(const BatchQueues &q, hipStream_t st) {
+  const int n = q.n > 3 ? q.n : 3;
+  hipLaunchKernelGGL(k_batch_reset, dim3((n + 255) / 256), dim3(256), 0, st, q.counters,
+                     q.maxw_all, q.n);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_forward(const BatchSliceDev *d, const BatchQueues &q, int it,
+                                int bundle_angles, bool exact, int grid, hipStream_t st) {
+  int ba_log2 = 0;
+  while ((1 << ba_log2) < bundle_angles && ba_log2 < 6) ++ba_log2;
+  if ((1 << ba_log2) != bundle_angles || grid <= 0) return hipErrorInvalidValue;
+  if (exact)
+    hipLaunchKernelGGL(k_forward_batch<true>, dim3(grid), dim3(kRayBlock), 0, st, d, q, it,
+                       ba_log2);
+  else
+    hipLaunchKernelGGL(k_forward_batch<false>, dim3(grid), dim3(kRayBlock), 0, st, d, q, it,
+                       ba_log2);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_backproject(const BatchSliceDev *d, const BatchQueues &q, int it,
+                                    int tw, int th, bool exact, int grid, size_t lds,
+                                    hipStream_t st) {
+  if (grid <= 0) return hipErrorInvalidValue;
+#define TRACE_BATCH_BP(TW_, TH_)                                                             \
+  if (tw == TW_ && th == TH_) {                                                              \
+    if (exact)                                                                               \
+      hipLaunchKernelGGL((k_bp_batch<TW_, TH_, true>), dim3(grid), dim3(kTileBlock), lds, st, \
+                         d, q, it);                                                          \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_bp_batch<TW_, TH_, false>), dim3(grid), dim3(kTileBlock), lds,    \
+                         st, d, q, it);                                                      \
+    return hipGetLastError();                                                                \
+  }
+  TRACE_BATCH_TILES(TRACE_BATCH_BP)
+#undef TRACE_BATCH_BP
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_batch_update(const BatchSliceDev *d, const BatchQueues &q, int it, int grid,
+                               hipStream_t st) {
+  if (grid <= 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_update_batch, dim3(grid), dim3(kPixBlock), 0, st, d, q, it);
+  return hipGetLastError();
+}
+
+bool batch_tile_supported(int tw, int th) {
+#define TRACE_BATCH_MATCH(TW_, TH_) \
+  if (tw == TW_ && th == TH_) return true;
+  TRACE_BATCH_TILES(TRACE_BATCH_MATCH)
+#undef TRACE_BATCH_MATCH
+  return false;
+}
+
+int batch_blocks_per_cu(int phase, int tw, int th, bool exact, size_t lds) {
+  int nb = 0;
+  hipError_t e = hipErrorInvalidValue;
+  if (phase == 0) {
+    e = exact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_forward_batch<true>,
+                                                             kRayBlock, 0)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_forward_batch<false>,
+                                                             kRayBlock, 0);
+  } else if (phase == 2) {
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_update_batch, kPixBlock, 0);
+  } else {
+#define TRACE_BATCH_OCC(TW_, TH_)                                                            \
+  if (tw == TW_ && th == TH_)                                                                \
+    e = exact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_bp_batch<TW_, TH_, true>, \
+                                                             kTileBlock, lds)                \
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(                                \
+                    &nb, k_bp_batch<TW_, TH_, false>, kTileBlock, lds);
+    TRACE_BATCH_TILES(TRACE_BATCH_OCC)
+#undef TRACE_BATCH_OCC
+  }
+  return e == hipSuccess ? nb : 0;
 }
 
 hipError_t launch_adjoint_atomic(const Geometry &g, const RayInfo *rays, int mode, bool exact,

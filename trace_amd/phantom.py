@@ -44,8 +44,9 @@ def mov_of(C: int, center: float) -> float:
     return float(mov)
 
 
-def phantom_slices(N: int, S: int, seed: int, K: int = 20, device="cpu"):
-    """Torch tensor [S][N][N] float32 of ellipse phantoms at pixel centres."""
+def phantom_slices(N: int, S: int, seed: int, K: int = 20, device="cpu", first_slice: int = 0):
+    """Torch tensor [S][N][N] float32 of ellipse phantoms at pixel centres
+    (slices first_slice .. first_slice + S - 1 of the seed)."""
     import torch
     dev = torch.device(device)
     half = N / 2.0
@@ -53,7 +54,7 @@ def phantom_slices(N: int, S: int, seed: int, K: int = 20, device="cpu"):
     Y, X = torch.meshgrid(coords, coords, indexing="ij")  # [iy][ix]
     out = torch.zeros((S, N, N), dtype=torch.float64, device=dev)
     for s in range(S):
-        for cx, cy, a, b, phi, rho in ellipse_params(seed, s, K):
+        for cx, cy, a, b, phi, rho in ellipse_params(seed, first_slice + s, K):
             c, sn = np.cos(phi), np.sin(phi)
             dx, dy = X - cx, Y - cy
             u = dx * c + dy * sn
@@ -63,9 +64,9 @@ def phantom_slices(N: int, S: int, seed: int, K: int = 20, device="cpu"):
 
 
 def line_integrals(theta, C: int, S: int, seed: int, K: int = 20,
-                   center: float | None = None, device="cpu"):
+                   center: float | None = None, device="cpu", first_slice: int = 0):
     """Exact ellipse line integrals [P][S][C] (float64 tensor, pixel units)
-    along the reference's rays."""
+    along the reference's rays (slices first_slice .. first_slice + S - 1)."""
     import torch
     dev = torch.device(device)
     N = C
@@ -79,7 +80,7 @@ def line_integrals(theta, C: int, S: int, seed: int, K: int = 20,
     out = torch.zeros((th.numel(), S, C), dtype=torch.float64, device=dev)
     for s in range(S):
         acc = torch.zeros((th.numel(), C), dtype=torch.float64, device=dev)
-        for cx, cy, a, b, phi, rho in ellipse_params(seed, s, K):
+        for cx, cy, a, b, phi, rho in ellipse_params(seed, first_slice + s, K):
             cxp, cyp, ap, bp = cx * half, cy * half, a * half, b * half
             c, sn = np.cos(phi), np.sin(phi)
             # normal in the ellipse frame and signed distance of the line
@@ -94,15 +95,16 @@ def line_integrals(theta, C: int, S: int, seed: int, K: int = 20,
 
 
 def noisy_sinogram(theta, C: int, S: int, seed: int, I0: float, K: int = 20,
-                   center: float | None = None, device="cpu"):
+                   center: float | None = None, device="cpu", first_slice: int = 0):
     """Poisson-noisy sinogram [P][S][C] float32 (non-negative)."""
     import torch
     dev = torch.device(device)
-    p = line_integrals(theta, C, S, seed, K, center, device)
+    p = line_integrals(theta, C, S, seed, K, center, device, first_slice)
     kappa = 2.0 / C
     lam = I0 * torch.exp(-kappa * p)
     gen = torch.Generator(device=dev)
-    gen.manual_seed(int(seed))
+    # noise stream per (seed, first slice); the seed alone for first_slice 0
+    gen.manual_seed(int(seed) if first_slice == 0 else int(seed) * 1000 + int(first_slice))
     counts = torch.poisson(lam, generator=gen)
     pt = -torch.log(torch.clamp(counts, min=1.0) / I0) / kappa
     return torch.clamp(pt, min=0.0).to(torch.float32)
