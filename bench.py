@@ -152,6 +152,11 @@ def main():
     return run_stack(args)
 
 
+def apply_shape(pj, tune):
+    from trace_amd.autotune import apply_shape as _apply
+    _apply(pj, tune)
+
+
 def tune_shape(theta, sino, N, S, alg, fast_sqrt, params=None, init=None):
     """Autotunes launch shapes on a sub-stack of at most 32 slices of the same
     geometry (the shapes depend on the geometry and the slice batches, not on
@@ -198,8 +203,7 @@ def run_stack(args):
     if not args.no_autotune:
         t0 = time.perf_counter()
         tune = tune_shape(theta, sino, N, S, args.alg, args.fast_sqrt)
-        pj.set_launch_shape(*tune["shape"])
-        pj.set_fwd_bundle(tune["bundle"])
+        apply_shape(pj, tune)
         log(f"autotuned shape {tune['shape']} bundle {tune['bundle']} "
             f"in {time.perf_counter() - t0:.1f} s")
 
@@ -295,6 +299,7 @@ def run_stack(args):
         "kernels": per_kernel,
         "launch_shape": {"fwd_batch_tile_w_h_batch": list(pj.launch_shape()),
                          "fwd_bundle_angles": tune["bundle"] if tune else 8,
+                         "fwd_lanes_per_ray": tune.get("lanes", 1) if tune else 1,
                          "autotuned": tune is not None},
         "phases_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in phases.items()},
         "cpu_baseline": cpu,
@@ -333,8 +338,7 @@ def run_regularised(args):
     tune = None
     if not args.no_autotune:
         tune = tune_shape(theta, sino, N, S, alg, args.fast_sqrt, prm, init=1.0)
-        pj.set_launch_shape(*tune["shape"])
-        pj.set_fwd_bundle(tune["bundle"])
+        apply_shape(pj, tune)
         log(f"autotuned shape {tune['shape']} bundle {tune['bundle']}")
     for w in range(args.warmup):
         pj.reconstruct(alg, 1, sino, recon.clone(), prm)
@@ -387,6 +391,7 @@ def run_regularised(args):
                                 / (elapsed / (iters * args.steps)) / 1e9 / HBM_PEAK_GBS,
         "launch_shape": {"fwd_batch_tile_w_h_batch": list(pj.launch_shape()),
                          "fwd_bundle_angles": tune["bundle"] if tune else 8,
+                         "fwd_lanes_per_ray": tune.get("lanes", 1) if tune else 1,
                          "autotuned": tune is not None},
         "checks": {"finite": bool(torch.isfinite(x).all().item()),
                    "nonnegative": bool((x >= 0).all().item())},
@@ -420,8 +425,7 @@ def run_autograd(args):
     tune = None
     if not args.no_autotune:
         tune = tune_shape(theta, y3, N, 1, "sirt", args.fast_sqrt)
-        op.pj.set_launch_shape(*tune["shape"])
-        op.pj.set_fwd_bundle(tune["bundle"])
+        apply_shape(op.pj, tune)
         log(f"autotuned shape {tune['shape']} bundle {tune['bundle']}")
     # adjoint dot-product test on this geometry
     g = torch.Generator(device="cuda").manual_seed(0)
@@ -511,6 +515,7 @@ def run_autograd(args):
                           "grad_finite": bool(torch.isfinite(p.grad).all().item())},
         "launch_shape": {"fwd_batch_tile_w_h_batch": list(op.pj.launch_shape()),
                          "fwd_bundle_angles": tune["bundle"] if tune else 8,
+                         "fwd_lanes_per_ray": tune.get("lanes", 1) if tune else 1,
                          "autotuned": tune is not None},
     }
     print(json.dumps(out), flush=True)
@@ -557,8 +562,7 @@ def run_batched(args):
     for s in slices:
         C = s.recon.shape[0]
         pj = Projector(s.theta, 1, C, s.center)
-        pj.set_launch_shape(*shapes[C]["shape"])
-        pj.set_fwd_bundle(shapes[C]["bundle"])
+        apply_shape(pj, shapes[C])
         x = s.recon.clone().reshape(1, C, C)
         pj.reconstruct(s.alg, 1, s.sino.reshape(-1, 1, C).contiguous(), x)  # tables, warm-up
         nnz_total += pj.count_segments()[0]
@@ -624,7 +628,8 @@ def run_batched(args):
                               for c in sorted({sp[0] for sp in specs})},
                    "sirt": sum(1 for sp in specs if sp[2] == "sirt")},
         "plain": {"seconds": t_plain, "slices_per_s": n / t_plain,
-                  "shapes": {str(k): {"shape": v["shape"], "bundle": v["bundle"]}
+                  "shapes": {str(k): {"shape": v["shape"], "bundle": v["bundle"],
+                                      "lanes": v.get("lanes", 1)}
                              for k, v in shapes.items()}},
         "batched": {"seconds": t_batch, "bundle": tune["bundle"], "tile": list(tune["tile"]),
                     "autotune_ms": tune["ms"]},

@@ -98,7 +98,12 @@ typedef struct trace_handle trace_handle_t;
                                      atomics (an independent check) */
 #define TRACE_OPT_FWD_BUNDLE 4 /* adjacent angles per forward-projector
                                   wavefront (1, 2, 4, 8, 16; the 64 lanes are
-                                  angles x 64/angles adjacent columns) */
+                                  angles x 64/(angles x lanes) adjacent
+                                  columns x lanes per ray) */
+#define TRACE_OPT_FWD_LANES 5 /* lanes per ray of the forward projector
+                                 (1 default, 2, 4): the lanes of a ray split
+                                 its slice batch; compiled for batches 8, 16
+                                 and 32 (2 lanes) and 16, 32 (4 lanes) */
 
 /* Kernel phases reported by trace_phase_times. */
 #define TRACE_PHASE_FORWARD 0 /* forward projection and per-ray weights */

@@ -14,7 +14,8 @@ pytestmark = pytest.mark.gpu
 
 SHAPES = [(16, 16, 16, 8), (4, 16, 16, 16), (8, 16, 16, 4), (32, 24, 24, 8), (4, 32, 16, 8),
           (4, 16, 32, 8), (16, 32, 32, 4), (8, 32, 32, 8), (4, 8, 8, 16), (4, 64, 64, 2),
-          (1, 32, 32, 1), (2, 64, 64, 1), (1, 32, 32, 2), (2, 16, 16, 2)]
+          (1, 32, 32, 1), (2, 64, 64, 1), (1, 32, 32, 2), (2, 16, 16, 2), (1, 16, 16, 1),
+          (16, 16, 16, 32), (16, 32, 32, 16), (8, 32, 16, 16), (4, 24, 24, 16)]
 
 
 def _oracle_counts(oracle, th, N, center):
@@ -161,3 +162,31 @@ def test_forward_bundles_bitexact(gpu, oracle, ba):
         pj.set_fwd_bundle(ba)
         y = pj.forward(torch.from_numpy(x).cuda()).cpu().numpy()
     assert np.array_equal(y.view(np.int32), oracle.forward(th, x, N / 2).view(np.int32))
+
+
+@pytest.mark.parametrize("fb,lanes,ba", [(8, 2, 8), (16, 2, 4), (32, 2, 16), (16, 4, 8),
+                                         (32, 4, 16), (16, 4, 1)])
+def test_forward_lanes_per_ray_bitexact(gpu, oracle, fb, lanes, ba):
+    """Splitting a ray's slice batch over 2 or 4 lanes leaves every ray sum
+    bit-identical (same segment order per slice)."""
+    torch = gpu
+    from trace_amd import Projector
+    N, P, S = 80, 30, 37
+    rng = np.random.default_rng(fb * lanes + ba)
+    th = rng.uniform(-1, 4, P).astype(np.float32)
+    x = rng.uniform(0, 1, (S, N, N)).astype(np.float32)
+    with Projector(th, S, N, N / 2) as pj:
+        pj.set_launch_shape(fb, 16, 16, 8)
+        pj.set_fwd_lanes(lanes)
+        pj.set_fwd_bundle(ba)
+        y = pj.forward(torch.from_numpy(x).cuda()).cpu().numpy()
+    assert np.array_equal(y.view(np.int32), oracle.forward(th, x, N / 2).view(np.int32))
+
+
+def test_forward_lanes_rejects_uncompiled(gpu):
+    from trace_amd import Projector, TraceUnsupported
+    th = np.linspace(0, 3, 10).astype(np.float32)
+    with Projector(th, 4, 32, 16.0) as pj:
+        pj.set_launch_shape(4, 16, 16, 4)
+        with pytest.raises(TraceUnsupported):
+            pj.set_fwd_lanes(2)  # batch 4 has no 2-lane form

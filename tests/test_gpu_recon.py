@@ -136,7 +136,8 @@ def test_nonfinite_sinogram_rejected(gpu):
 
 @pytest.mark.parametrize("shape", [(4, 64, 64, 2), (8, 32, 32, 8), (32, 16, 16, 16),
                                    (16, 16, 16, 8), (16, 24, 24, 8), (1, 32, 32, 1),
-                                   (2, 64, 64, 1), (2, 16, 16, 2)])
+                                   (2, 64, 64, 1), (2, 16, 16, 2), (16, 16, 16, 32, 2),
+                                   (16, 32, 32, 16, 4), (8, 32, 16, 16, 2)])
 @pytest.mark.parametrize("alg", ["sirt", "mlem"])
 def test_launch_shapes_recon(gpu, oracle, alg, shape):
     """Every compiled launch shape gives the same reconstruction."""
@@ -148,7 +149,9 @@ def test_launch_shapes_recon(gpu, oracle, alg, shape):
     sino = noisy_sinogram(th, N, S, seed=7, I0=1e4, K=6).numpy()
     init = np.full((S, N, N), 0.0 if alg == "sirt" else 1.0, np.float32)
     with Projector(th, S, N, N / 2) as pj:
-        pj.set_launch_shape(*shape)
+        pj.set_launch_shape(*shape[:4])
+        if len(shape) > 4:
+            pj.set_fwd_lanes(shape[4])  # lanes per ray
         dx = torch.from_numpy(init).cuda()
         pj.reconstruct(alg, iters, torch.from_numpy(sino).cuda(), dx)
         torch.cuda.synchronize()

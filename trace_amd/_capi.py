@@ -27,6 +27,7 @@ TRACE_OPT_EXACT_SQRT = 1
 TRACE_OPT_PHASE_TIMING = 2
 TRACE_OPT_BACKPROJECTOR = 3
 TRACE_OPT_FWD_BUNDLE = 4
+TRACE_OPT_FWD_LANES = 5
 PHASES = ("forward", "back", "update", "setup")
 
 

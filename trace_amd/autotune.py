@@ -18,7 +18,8 @@ FWD_BATCHES = (1, 2, 4, 8, 16, 32)
 BUNDLES = (1, 2, 4, 8, 16)
 TILES = ((16, 16, 8), (16, 16, 16), (16, 16, 4), (24, 24, 8), (32, 16, 8),
          (16, 32, 8), (32, 32, 4), (32, 32, 8), (8, 8, 16), (64, 64, 2),
-         (32, 32, 1), (64, 64, 1), (32, 32, 2), (16, 16, 2))
+         (32, 32, 1), (64, 64, 1), (32, 32, 2), (16, 16, 2), (16, 16, 1),
+         (16, 16, 32), (32, 32, 16), (32, 16, 16), (24, 24, 16))
 
 
 def _pow2_ceil(n: int) -> int:
@@ -39,6 +40,23 @@ def batch_candidates(batches, S: int):
 _CACHE: dict = {}
 
 
+# (slice batch, lanes per ray) pairs compiled into the forward projector
+FWD_SHAPES = {(1, 1), (2, 1), (4, 1), (8, 1), (16, 1), (32, 1), (8, 2), (16, 2), (32, 2),
+              (16, 4), (32, 4)}
+LANES = (1, 2, 4)
+
+
+def apply_shape(pj, res: dict):
+    """Sets {"shape": (fwd_batch, tile_w, tile_h, tile_batch), "bundle",
+    "lanes"} on projector `pj` (lanes first reset to 1 so that every
+    compiled batch is accepted)."""
+    pj.set_fwd_lanes(1)
+    pj.set_fwd_bundle(1)
+    pj.set_launch_shape(*res["shape"])
+    pj.set_fwd_lanes(res.get("lanes", 1))
+    pj.set_fwd_bundle(res["bundle"])
+
+
 def _key(pj, alg):
     import torch
     name = torch.cuda.get_device_name(pj.device)
@@ -47,10 +65,12 @@ def _key(pj, alg):
 
 def autotune(pj, sino, recon, alg: str = "mlem", params=None,
              fwd_batches=FWD_BATCHES, bundles=BUNDLES, tiles=TILES,
-             cache_file: str | None = None, verbose: bool = False) -> dict:
+             cache_file: str | None = None, verbose: bool = False,
+             lanes_per_ray=LANES) -> dict:
     """Times the candidates on (sino, a copy of recon) and sets the best
     shape on projector `pj`.  Returns {"shape": (fb, tw, th, tb),
-    "bundle": angles, "forward_ms": {...}, "back_ms": {...}}."""
+    "bundle": angles, "lanes": lanes per ray, "forward_ms": {...},
+    "back_ms": {...}}."""
     import torch
     key = _key(pj, alg)
     if cache_file and not _CACHE and os.path.exists(cache_file):
@@ -61,8 +81,7 @@ def autotune(pj, sino, recon, alg: str = "mlem", params=None,
             pass
     if key in _CACHE:
         res = _CACHE[key]
-        pj.set_launch_shape(*res["shape"])
-        pj.set_fwd_bundle(res["bundle"])
+        apply_shape(pj, res)
         return res
     work = recon.clone()
     fwd_ms, back_ms = {}, {}
@@ -71,24 +90,28 @@ def autotune(pj, sino, recon, alg: str = "mlem", params=None,
     pj.set_phase_timing(True)
     try:
         for fb in batch_candidates(fwd_batches, pj.S):
-            for ba in bundles:
-                pj.set_launch_shape(fb, *base_tile)
-                pj.set_fwd_bundle(ba)
-                work.copy_(recon)
-                pj.reconstruct(alg, 1, sino, work, params)  # allocation, warm caches
-                pj.phase_times(reset=True)
-                work.copy_(recon)
-                pj.reconstruct(alg, 1, sino, work, params)
-                fwd_ms[(fb, ba)] = pj.phase_times(reset=True)["forward"][0]
-                if verbose:
-                    print(f"[autotune] forward batch {fb} bundle {ba}: "
-                          f"{fwd_ms[(fb, ba)]:.2f} ms", flush=True)
-        best_fb, best_ba = min(fwd_ms, key=fwd_ms.get)
-        pj.set_fwd_bundle(best_ba)
+            for lanes in lanes_per_ray:
+                if (fb, lanes) not in FWD_SHAPES:
+                    continue
+                for ba in bundles:
+                    if ba * lanes > 64:
+                        continue
+                    apply_shape(pj, {"shape": (fb, *base_tile), "bundle": ba, "lanes": lanes})
+                    work.copy_(recon)
+                    pj.reconstruct(alg, 1, sino, work, params)  # allocation, warm caches
+                    pj.phase_times(reset=True)
+                    work.copy_(recon)
+                    pj.reconstruct(alg, 1, sino, work, params)
+                    fwd_ms[(fb, lanes, ba)] = pj.phase_times(reset=True)["forward"][0]
+                    if verbose:
+                        print(f"[autotune] forward batch {fb} lanes {lanes} bundle {ba}: "
+                              f"{fwd_ms[(fb, lanes, ba)]:.2f} ms", flush=True)
+        best_fb, best_lanes, best_ba = min(fwd_ms, key=fwd_ms.get)
+        apply_shape(pj, {"shape": (best_fb, *base_tile), "bundle": best_ba, "lanes": best_lanes})
         for t in tiles:
             if t[2] not in tile_batches:
                 continue
-            pj.set_launch_shape(best_fb, *t)
+            apply_shape(pj, {"shape": (best_fb, *t), "bundle": best_ba, "lanes": best_lanes})
             work.copy_(recon)
             pj.reconstruct(alg, 1, sino, work, params)
             back_ms[t] = pj.phase_times(reset=True)["back"][0]
@@ -99,10 +122,10 @@ def autotune(pj, sino, recon, alg: str = "mlem", params=None,
         pj.set_phase_timing(False)
         torch.cuda.synchronize(pj.device)
     shape = (best_fb, *best_tile)
-    pj.set_launch_shape(*shape)
-    res = {"shape": list(shape), "bundle": best_ba,
-           "forward_ms": {f"{k[0]}x{k[1]}": v for k, v in fwd_ms.items()},
+    res = {"shape": list(shape), "bundle": best_ba, "lanes": best_lanes,
+           "forward_ms": {f"{k[0]}x{k[1]}x{k[2]}": v for k, v in fwd_ms.items()},
            "back_ms": {"x".join(map(str, k)): v for k, v in back_ms.items()}}
+    apply_shape(pj, res)
     _CACHE[key] = res
     if cache_file:
         try:

@@ -93,7 +93,8 @@ int prepare(trace_batch *b) {
     if (int rc = ensure_count(h)) return rc;
     maxP = std::max(maxP, h->geo.P);
   }
-  b->lds = (size_t)b->th * (b->tw + 1) * sizeof(unsigned long long) + (size_t)maxP * sizeof(int);
+  // 32-bit accumulators of one slice, then the per-angle column counts
+  b->lds = (size_t)b->th * (b->tw + 1) * sizeof(unsigned) + (size_t)maxP * sizeof(int);
   if (b->lds > 64 * 1024)
     return fail(TRACE_E_UNSUPPORTED, "tile and angle count exceed the batched kernel's LDS");
   // per-phase order (decreasing cost per item) and start tables
@@ -136,6 +137,7 @@ int prepare(trace_batch *b) {
     d.iters = b->s[i].iters;
     d.yi0 = h->geo.yi0;
     d.maxlen = h->maxlen;
+    d.denmax = h->den_max;
     d.sino = b->s[i].sino;
     d.x = b->s[i].recon;
     d.w = h->w;

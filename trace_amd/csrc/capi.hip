@@ -84,7 +84,7 @@ int backproject(trace_handle *h, int mode, const float *w, float *z, int phase) 
   if (h->backprojector == 0) {
     if (int rc = ensure_count(h)) return rc;
     TIMED(phase, launch_backproject(h->geo, h->rays, h->count, h->tile, mode, h->exact, w,
-                                    h->maxw, h->maxlen, z, h->stream));
+                                    h->maxw, h->maxlen, h->den_max, z, h->stream));
   } else {
     const int ns = (mode == kBpOutSino) ? h->geo.S : 1;
     TIMED(phase, launch_fill(z, (int64_t)ns * h->geo.NN(), 0.f, h->stream));
@@ -96,6 +96,19 @@ int backproject(trace_handle *h, int mode, const float *w, float *z, int phase) 
 int ensure_den(trace_handle *h) {
   if (h->den_ready && h->den_exact == h->exact) return TRACE_OK;
   if (int rc = backproject(h, kBpOutOnes, nullptr, h->den, TRACE_PHASE_SETUP)) return rc;
+  // largest column sum: bounds every pixel's sum of segment lengths, which
+  // sets the 32-bit fixed-point scale of the weights backprojector
+  unsigned *bits = reinterpret_cast<unsigned *>(h->flag);
+  TIMED(TRACE_PHASE_SETUP, launch_fill(reinterpret_cast<float *>(bits), 1, 0.f, h->stream));
+  TIMED(TRACE_PHASE_SETUP,
+        launch_absmax_slices(h->den, 1, 1, (int)h->geo.NN(), bits, h->stream));
+  unsigned b = 0;
+  HIP_TRY(hipMemcpyAsync(&b, bits, sizeof(b), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  float m = 0.f;
+  std::memcpy(&m, &b, sizeof(m));
+  // den itself carries a fixed-point error far below 2^-20 relative
+  h->den_max = std::isfinite(m) && m > 0.f ? m * (1.0f + 0x1p-16f) : 1.0f;
   h->den_ready = true;
   h->den_exact = h->exact;
   return TRACE_OK;
@@ -153,6 +166,14 @@ float mov_of(int C, float center) {
 }
 
 // Launch shapes compiled into the library (kernels.hip, launch_backproject).
+bool fwd_shape_supported(int sb, int lanes) {
+#define TRACE_FWD_MATCH(SB_, LPR_) \
+  if (sb == SB_ && lanes == LPR_) return true;
+  TRACE_FWD_SHAPES(TRACE_FWD_MATCH)
+#undef TRACE_FWD_MATCH
+  return false;
+}
+
 bool tile_shape_supported(int tw, int th, int sb) {
 #define TRACE_SHAPE_MATCH(TW_, TH_, SB_) \
   if (tw == TW_ && th == TH_ && sb == SB_) return true;
@@ -168,7 +189,8 @@ int forward(trace_handle *h, int mode, const float *sino, float *out) {
   if (mode != kFwdOutY)
     TIMED(TRACE_PHASE_SETUP,
           launch_fill(reinterpret_cast<float *>(h->maxw), h->geo.S, 0.f, h->stream));
-  TIMED(TRACE_PHASE_FORWARD, launch_forward(h->geo, h->rays, h->fwd_sb, h->fwd_bundle, mode,
+  TIMED(TRACE_PHASE_FORWARD, launch_forward(h->geo, h->rays, h->fwd_sb, h->fwd_bundle,
+                                            h->fwd_lanes, mode,
                                             h->exact, h->xb, sino, out, h->maxw, h->stream));
   return TRACE_OK;
 }
@@ -338,7 +360,16 @@ int trace_set_option(trace_handle_t *h, int option, int value) {
     case TRACE_OPT_FWD_BUNDLE:
       if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
         return fail(TRACE_E_INVALID, "angles per bundle must be 1, 2, 4, 8 or 16");
+      if (value * h->fwd_lanes > 64)
+        return fail(TRACE_E_INVALID, "angles per bundle x lanes per ray exceeds 64");
       h->fwd_bundle = value;
+      return TRACE_OK;
+    case TRACE_OPT_FWD_LANES:
+      if (value != 1 && value != 2 && value != 4)
+        return fail(TRACE_E_INVALID, "lanes per ray must be 1, 2 or 4");
+      if (!fwd_shape_supported(h->fwd_sb, value))
+        return fail(TRACE_E_UNSUPPORTED, "slice batch and lanes per ray not compiled");
+      h->fwd_lanes = value;
       return TRACE_OK;
     default:
       return fail(TRACE_E_UNSUPPORTED, "unknown option");
@@ -352,6 +383,9 @@ int trace_set_launch_shape(trace_handle_t *h, int fwd_batch, int tile_w, int til
   if (fwd_batch != 1 && fwd_batch != 2 && fwd_batch != 4 && fwd_batch != 8 &&
       fwd_batch != 16 && fwd_batch != 32)
     return fail(TRACE_E_UNSUPPORTED, "forward slice batch must be 1, 2, 4, 8, 16 or 32");
+  if (!fwd_shape_supported(fwd_batch, h->fwd_lanes))
+    return fail(TRACE_E_UNSUPPORTED,
+                "forward slice batch not compiled for the handle's lanes per ray");
   if (!tile_shape_supported(tile_w, tile_h, tile_batch))
     return fail(TRACE_E_UNSUPPORTED, "tile shape not compiled into this library");
   h->fwd_sb = fwd_batch;
@@ -493,6 +527,7 @@ int trace_adjoint(trace_handle_t *h, const float *r, float *z) {
   if (!r || !z) return fail(TRACE_E_INVALID, "null buffer");
   HIP_TRY(hipSetDevice(h->device));
   if (int rc = ensure_work(h, false)) return rc;
+  if (int rc = ensure_den(h)) return rc;  // fixed-point scale of A^T
   TIMED(TRACE_PHASE_SETUP,
         launch_fill(reinterpret_cast<float *>(h->maxw), h->geo.S, 0.f, h->stream));
   TIMED(TRACE_PHASE_SETUP,

@@ -100,6 +100,7 @@ struct trace_handle {
   int backprojector = 0;  // 0 tile (LDS), 1 whole-ray global atomics
   int fwd_sb = 16;        // slices per forward lane (xb layout)
   int fwd_bundle = 8;     // adjacent angles per forward wavefront
+  int fwd_lanes = 1;      // lanes per ray of the forward projector
   trace_hip::TileShape tile{16, 16, 16};
   trace_hip::RayInfo *rays = nullptr;     // [P][C]
   bool rays_ready = false;
@@ -118,6 +119,7 @@ struct trace_handle {
   unsigned long long *counters = nullptr;  // [3]: segments, outside image, max length bits
   unsigned *maxw = nullptr;    // [S] bits of max |w| per slice
   float maxlen = 0.f;          // longest segment of the geometry
+  float den_max = 0.f;         // bound on the sum of segment lengths per pixel
   int64_t nnz = 0, nnz_oor = 0;
   trace_hip::PhaseTimer timer;
 };

@@ -54,8 +54,8 @@ int tile_count_bound(float sinq, float cosq, int TW, int TH);
 // Launchers (kernels.hip).  All are asynchronous on `st`.
 hipError_t launch_ray_info(const Geometry &g, RayInfo *rays, hipStream_t st);
 hipError_t launch_forward(const Geometry &g, const RayInfo *rays, int sb, int bundle_angles,
-                          int mode, bool exact, const float *xb, const float *sino,
-                          float *out, unsigned *maxw, hipStream_t st);
+                          int lanes_per_ray, int mode, bool exact, const float *xb,
+                          const float *sino, float *out, unsigned *maxw, hipStream_t st);
 // One slice of a batched reconstruction: its own geometry (P, C = N,
 // centre), tables and buffers; one slice per geometry.
 struct BatchSliceDev {
@@ -68,6 +68,7 @@ struct BatchSliceDev {
   int iters;
   float yi0;
   float maxlen;           // longest segment of the geometry
+  float denmax;           // bound on the sum of lengths per pixel (max of den)
   const float *sino;      // [P][C]
   float *x;               // [N][N] image (updated in place)
   float *w;               // [P][C] per-ray weights
@@ -91,16 +92,24 @@ struct BatchQueues {
 // Compiled tile shapes of the batched backprojector (one slice per tile).
 #define TRACE_BATCH_TILES(X) X(16, 16) X(32, 32) X(64, 64)
 
+// Forward projector shapes compiled into the library: (slices per lane
+// batch, lanes per ray).
+#define TRACE_FWD_SHAPES(X)                                                \
+  X(1, 1) X(2, 1) X(4, 1) X(8, 1) X(16, 1) X(32, 1) X(8, 2) X(16, 2) X(32, 2) \
+  X(16, 4) X(32, 4)
+
 // Tile shapes (tile_w, tile_h, slices per workgroup) compiled into the
 // backprojector; the C-ABI accepts exactly these.
 #define TRACE_BP_SHAPES(X)                                                 \
   X(16, 16, 8) X(16, 16, 16) X(16, 16, 4) X(24, 24, 8) X(32, 16, 8)          \
   X(16, 32, 8) X(32, 32, 4) X(32, 32, 8) X(8, 8, 16) X(64, 64, 2)           \
-  X(32, 32, 1) X(64, 64, 1) X(32, 32, 2) X(16, 16, 2) X(16, 16, 1)
+  X(32, 32, 1) X(64, 64, 1) X(32, 32, 2) X(16, 16, 2) X(16, 16, 1)          \
+  X(16, 16, 32) X(32, 32, 16) X(32, 16, 16) X(24, 24, 16)
 
 hipError_t launch_backproject(const Geometry &g, const RayInfo *rays, const int *count,
                               const TileShape &ts, int mode, bool exact, const float *w,
-                              const unsigned *maxw, float maxlen, float *z, hipStream_t st);
+                              const unsigned *maxw, float maxlen, float denmax, float *z,
+                              hipStream_t st);
 hipError_t launch_absmax_slices(const float *w, int P, int S, int C, unsigned *maxw,
                                 hipStream_t st);
 hipError_t launch_to_pcs(const float *r, int P, int S, int C, float *w, unsigned *maxw,

@@ -25,6 +25,8 @@
 // disp_engine_reduction.h:104-185).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "internal.h"
 
 #pragma clang fp contract(off)
@@ -100,7 +102,11 @@ __device__ __forceinline__ void store_slices(float *__restrict__ p, const float 
 //   EM    w = p/y                           (mlem.cc:50, pml.cc:236)
 // to w[p][c][s] (slices innermost, read as a block by the backprojector) and
 // fold max |w| per slice into maxw (bits; zeroed beforehand).
-template <int SB, int MODE, bool kExact>
+// LPR lanes per ray (1, 2 or 4): the lanes of one ray repeat its walk and
+// each gathers and sums SB/LPR of the batch's slices, so a gather touches
+// LPR-fold fewer distinct cache lines per wavefront instruction (the
+// address unit, not the VALU, limits the one-lane form).
+template <int SB, int MODE, bool kExact, int LPR = 1>
 __device__ __forceinline__ void forward_bundle(const GridInfo &g, const AngleInfo *__restrict__ ang,
                                                const RayInfo *__restrict__ rays, int P, int S,
                                                int ba_log2, const float *__restrict__ xb,
@@ -108,18 +114,24 @@ __device__ __forceinline__ void forward_bundle(const GridInfo &g, const AngleInf
                                                float *__restrict__ out,
                                                unsigned *__restrict__ maxw, int64_t bundle,
                                                int b) {
+  static_assert(LPR == 1 || LPR == 2 || LPR == 4, "lanes per ray: 1, 2 or 4");
+  static_assert(SB % LPR == 0, "slice batch must split evenly over the lanes of a ray");
+  constexpr int SBL = SB / LPR;  // slices per lane
+  constexpr int LPR_LOG2 = LPR == 1 ? 0 : (LPR == 2 ? 1 : 2);
   const int C = g.C;
   const int lane = threadIdx.x & 63;
-  const int bc_log2 = 6 - ba_log2;  // columns per bundle = 64 / BA
+  const int sub = lane & (LPR - 1);    // slice group of this lane
+  const int rl = lane >> LPR_LOG2;     // ray of this lane within the wavefront
+  const int bc_log2 = 6 - LPR_LOG2 - ba_log2;  // columns per bundle = 64 / (LPR * BA)
   const int nbc = (C + (1 << bc_log2) - 1) >> bc_log2;
-  const int p = (int)(bundle / nbc) * (1 << ba_log2) + (lane >> bc_log2);
-  const int c = (int)(bundle % nbc) * (1 << bc_log2) + (lane & ((1 << bc_log2) - 1));
+  const int p = (int)(bundle / nbc) * (1 << ba_log2) + (rl >> bc_log2);
+  const int c = (int)(bundle % nbc) * (1 << bc_log2) + (rl & ((1 << bc_log2) - 1));
   const bool valid = (p < P) && (c < C);
   const unsigned NN = (unsigned)g.N * (unsigned)g.N;
-  const float *xbb = xb + (int64_t)b * NN * SB;
-  float acc[SB];
+  const float *xbb = xb + (int64_t)b * NN * SB + sub * SBL;
+  float acc[SBL];
 #pragma unroll
-  for (int k = 0; k < SB; ++k) acc[k] = 0.f;
+  for (int k = 0; k < SBL; ++k) acc[k] = 0.f;
   float a2 = 0.f;
   const int64_t t = (int64_t)p * C + c;
   if (valid) {
@@ -127,21 +139,21 @@ __device__ __forceinline__ void forward_bundle(const GridInfo &g, const AngleInf
     traverse_full<kExact>(g, r, ang[p].quadrant, [&](int ix, int iy, float l, float l2) {
       const unsigned idx = (unsigned)(ix + iy * g.N);
       if (idx < NN) {
-        float v[SB];
-        load_slices<SB>(xbb + (int64_t)idx * SB, v);
+        float v[SBL];
+        load_slices<SBL>(xbb + (int64_t)idx * SB, v);
 #pragma unroll
-        for (int k = 0; k < SB; ++k) acc[k] += v[k] * l;
+        for (int k = 0; k < SBL; ++k) acc[k] += v[k] * l;
       }
       if (MODE == kFwdSIRT) a2 += l2;
     });
   }
 #pragma unroll
-  for (int k = 0; k < SB; ++k) {
-    const int s = b * SB + k;
-    if (s >= S) break;  // uniform across the block
-    const int64_t o = ((int64_t)p * S + s) * C + c;
+  for (int k = 0; k < SBL; ++k) {
+    const int s = b * SB + sub * SBL + k;
+    const bool live = valid && (s < S);
     unsigned m = 0;
-    if (valid) {
+    if (live) {
+      const int64_t o = ((int64_t)p * S + s) * C + c;
       if (MODE == kFwdY) {
         out[o] = acc[k];
       } else {
@@ -151,21 +163,24 @@ __device__ __forceinline__ void forward_bundle(const GridInfo &g, const AngleInf
       }
     }
     if (MODE != kFwdY) {
-      for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_down((int)m, off, 64));
-      if (lane == 0) atomicMax(maxw + s, m);
+      // max over the lanes holding the same slice (same sub)
+      for (int off = 32; off >= LPR; off >>= 1)
+        m = max(m, (unsigned)__shfl_down((int)m, off, 64));
+      if (lane < LPR && s < S) atomicMax(maxw + s, m);
     }
   }
 }
 
-template <int SB, int MODE, bool kExact>
+template <int SB, int MODE, bool kExact, int LPR>
 __global__ void __launch_bounds__(kRayBlock)
 k_forward(GridInfo g, const AngleInfo *__restrict__ ang, const RayInfo *__restrict__ rays,
           int P, int S, int ba_log2, const float *__restrict__ xb,
           const float *__restrict__ sino, float *__restrict__ out,
           unsigned *__restrict__ maxw) {
-  forward_bundle<SB, MODE, kExact>(g, ang, rays, P, S, ba_log2, xb, sino, out, maxw,
-                                   (int64_t)blockIdx.x * (kRayBlock / 64) + (threadIdx.x >> 6),
-                                   blockIdx.y);
+  forward_bundle<SB, MODE, kExact, LPR>(g, ang, rays, P, S, ba_log2, xb, sino, out, maxw,
+                                        (int64_t)blockIdx.x * (kRayBlock / 64) +
+                                            (threadIdx.x >> 6),
+                                        blockIdx.y);
 }
 
 // w[p][c][s] = r[p][s][c], folding max |r| per slice into maxw (zeroed
@@ -201,6 +216,7 @@ struct TileArgs {
   const float *w;     // [P][C][S] weights (kBpSino)
   const unsigned *maxw;  // [S] bits of max |w| per slice (kBpSino)
   float maxlen;       // longest segment of the geometry
+  float denmax;       // bound on sum of lengths per pixel (max of A^T 1), kBpSino
   float *z;           // [S][N][N] output (kBpOnes/kBpCount: [N][N])
 };
 
@@ -232,16 +248,21 @@ struct TilePitch {
 
 // Fixed-point accumulation.  LDS float atomics (ds_add_f32) run at a small
 // fraction of the rate of LDS integer atomics on gfx950, so each term len*w
-// is scaled by 2^e (e per slice, such that |len*w| * 2^e < 2^30 for every
-// term: from the slice's largest |w| and the geometry's longest segment),
-// converted to an integer and added with ds_add_u64.  Integer addition is
-// associative, so the result does not depend on the order in which lanes
-// and waves arrive: the backprojection is bit-reproducible, and its error
-// (below one unit of 2^-e per term, i.e. 2^-30 of the largest term) is far
-// below that of an fp32 sum.
+// is scaled by 2^e, converted to an integer and added with an integer LDS
+// atomic.  Integer addition is associative, so the result does not depend
+// on the order in which lanes and waves arrive: the backprojection is
+// bit-reproducible.
+//   weights (kBpSino): 32-bit words, e per slice such that
+//     2^e * max|w| * denmax < 2^30, denmax >= the largest sum of segment
+//     lengths in one pixel (max of A^T 1); every pixel's sum then stays
+//     below 2^30 + (number of terms)/2, so it cannot overflow.  Terms are
+//     rounded to nearest (unbiased; error <= 2^-e/2 per term, about 2^-31
+//     of max|w| * denmax).
+//   column sums (kBpOnes): 64-bit words, e from the longest segment
+//     (|term| < 2^30), truncated.
 // A slice whose weights are not all finite (y = 0 rays give inf/NaN in the
-// reference too) is accumulated in float instead, in the low word of the
-// same slot, so that inf/NaN propagate as they do in the reference.
+// reference too) is accumulated in float instead, in the same slot, so that
+// inf/NaN propagate as they do in the reference.
 __device__ __forceinline__ int fixed_exponent(float maxw, float maxlen, bool &fixed) {
   const float m = maxw * maxlen;
   fixed = __builtin_isfinite(m);
@@ -250,13 +271,19 @@ __device__ __forceinline__ int fixed_exponent(float maxw, float maxlen, bool &fi
   return 29 - ilogbf(m);
 }
 
+// Accumulator word of a mode: the weights backprojector sums 32-bit fixed
+// point (or float bits for a non-finite slice); column sums and counts use
+// 64-bit words.
+template <int MODE>
+using BpWord = typename std::conditional<MODE == kBpSino, unsigned, unsigned long long>::type;
+
 // Main loop of the tile backprojector: lanes take consecutive (angle,
 // column) pairs of the flattened per-angle column ranges and add their
 // segments near the tile into the LDS accumulators.  ALLFX: every slice of
 // the batch uses the fixed-point path (the common case; no per-slice
 // branch in the inner loop).
 template <int TW, int TH, int SB, int MODE, bool kExact, bool ALLFX>
-__device__ __forceinline__ void bp_tile_loop(const TileArgs &a, unsigned long long *acc,
+__device__ __forceinline__ void bp_tile_loop(const TileArgs &a, BpWord<MODE> *acc,
                                              const int *count, int tx0, int tx1, int ty0,
                                              int ty1, int b, const int (&ex)[SB],
                                              const bool (&fx)[SB]) {
@@ -302,16 +329,18 @@ __device__ __forceinline__ void bp_tile_loop(const TileArgs &a, unsigned long lo
           const unsigned lx = (unsigned)(ix - tx0);
           const unsigned ly = (unsigned)(iy - ty0);
           if (lx < (unsigned)(tx1 - tx0) && ly < (unsigned)(ty1 - ty0)) {
-            unsigned long long *dst = acc + ly * PITCH + lx;
+            BpWord<MODE> *dst = acc + ly * PITCH + lx;
             if (MODE == kBpCount) {
-              atomicAdd(dst, 1ull);
+              atomicAdd(dst, (BpWord<MODE>)1);
+            } else if (MODE == kBpOnes) {
+              // |t| < 2^30: truncating conversion, error < 2^-e
+              atomicAdd(dst, (BpWord<MODE>)(long long)(int)(l * wv[0]));
             } else {
 #pragma unroll
               for (int q = 0; q < SB; ++q) {
                 const float t = l * wv[q];
                 if (ALLFX || fx[q]) {
-                  // |t| < 2^30: truncating conversion, error < 2^-e
-                  atomicAdd(dst + q * PLANE, (unsigned long long)(long long)(int)t);
+                  atomicAdd(dst + q * PLANE, (BpWord<MODE>)(int)__builtin_rintf(t));
                 } else {
                   atomicAdd(reinterpret_cast<float *>(dst + q * PLANE), t);
                 }
@@ -328,12 +357,13 @@ __device__ __forceinline__ void bp_tile_loop(const TileArgs &a, unsigned long lo
 
 // Workgroup (tile, slice batch): LDS accumulators acc[SB][TH][TW + 1] for
 // all angles, then one coalesced write-out.  `acc` is the workgroup's
-// dynamic LDS ([SB][TH][TW + 1] 64-bit words, then int count[P] when
-// a.count_lds); the body starts and ends with a workgroup barrier, so a
+// dynamic LDS ([SB][TH][TW + 1] words of BpWord<MODE>, then int count[P]
+// when a.count_lds); the body starts and ends with a workgroup barrier, so a
 // persistent workgroup may run it for one tile after another.
 template <int TW, int TH, int SB, int MODE, bool kExact>
 __device__ __forceinline__ void bp_tile_body(const TileArgs &a, int tile, int b,
-                                             unsigned long long *acc) {
+                                             unsigned long long *lds) {
+  BpWord<MODE> *acc = reinterpret_cast<BpWord<MODE> *>(lds);
   constexpr int PITCH = TilePitch<TW, TH>::kPitch;
   constexpr int PLANE = TilePitch<TW, TH>::kPlane;
   int *lcount = reinterpret_cast<int *>(acc + SB * PLANE);
@@ -344,7 +374,7 @@ __device__ __forceinline__ void bp_tile_body(const TileArgs &a, int tile, int b,
   const int tx0 = tx * TW, ty0 = ty * TH;
   const int tx1 = min(tx0 + TW, N), ty1 = min(ty0 + TH, N);
   __syncthreads();  // the previous tile's write-out has read acc
-  for (int i = threadIdx.x; i < SB * PLANE; i += kTileBlock) acc[i] = 0ull;
+  for (int i = threadIdx.x; i < SB * PLANE; i += kTileBlock) acc[i] = 0;
   if (a.count_lds)
     for (int i = threadIdx.x; i < a.P; i += kTileBlock) lcount[i] = a.count[i];
   // per-slice scale exponents (uniform across the workgroup)
@@ -354,9 +384,10 @@ __device__ __forceinline__ void bp_tile_body(const TileArgs &a, int tile, int b,
 #pragma unroll
   for (int q = 0; q < SB; ++q) {
     const int s = b * SB + q;
-    float m = 1.0f;
-    if (MODE == kBpSino) m = (s < a.S) ? __uint_as_float(a.maxw[s]) : 0.f;
-    ex[q] = fixed_exponent(m, a.maxlen, fx[q]);
+    if (MODE == kBpSino)
+      ex[q] = fixed_exponent((s < a.S) ? __uint_as_float(a.maxw[s]) : 0.f, a.denmax, fx[q]);
+    else
+      ex[q] = fixed_exponent(1.0f, a.maxlen, fx[q]);
     all_fixed &= fx[q];
   }
   __syncthreads();
@@ -374,14 +405,16 @@ __device__ __forceinline__ void bp_tile_body(const TileArgs &a, int tile, int b,
     float *zs = a.z + (int64_t)s * N * N;
     for (int i = threadIdx.x; i < w_ * h_; i += kTileBlock) {
       const int ly = i / w_, lx = i - ly * w_;
-      const unsigned long long v = acc[q * PLANE + ly * PITCH + lx];
+      const BpWord<MODE> v = acc[q * PLANE + ly * PITCH + lx];
       float out;
       if (MODE == kBpCount)
         out = (float)v;
-      else if (fx[q])
-        out = (float)ldexp((double)(long long)v, -ex[q]);
-      else
+      else if (!fx[q])
         out = __uint_as_float((unsigned)v);
+      else if (MODE == kBpSino)
+        out = (float)ldexp((double)(int)v, -ex[q]);
+      else
+        out = (float)ldexp((double)(long long)v, -ex[q]);
       zs[(int64_t)(ty0 + ly) * N + tx0 + lx] = out;
     }
   }
@@ -705,6 +738,7 @@ k_bp_batch(const BatchSliceDev *__restrict__ d, BatchQueues q, int it) {
     a.w = sd.w;
     a.maxw = sd.maxw;
     a.maxlen = sd.maxlen;
+    a.denmax = sd.denmax;
     a.z = sd.z;
     bp_tile_body<TW, TH, 1, kBpSino, kExact>(a, item - start[k], 0, acc);
   }
@@ -847,23 +881,32 @@ inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b
 
 // ------------------------------------------------ launch-shape dispatch --
 
-template <int SB, bool E>
+template <int SB, bool E, int LPR>
 hipError_t fwd_mode(int mode, const Geometry &g, const RayInfo *rays, int ba_log2,
                     const float *xb, const float *sino, float *out, unsigned *maxw,
                     hipStream_t st) {
-  const int bc = 64 >> ba_log2, ba = 1 << ba_log2;
+  const int bc = (64 / LPR) >> ba_log2, ba = 1 << ba_log2;
+  if (bc < 1) return hipErrorInvalidValue;
   const int64_t bundles = (int64_t)((g.P + ba - 1) / ba) * ((g.C + bc - 1) / bc);
   dim3 grid(blocks_for(bundles, kRayBlock / 64), (g.S + SB - 1) / SB), block(kRayBlock);
   if (mode == kFwdY)
-    hipLaunchKernelGGL((k_forward<SB, kFwdY, E>), grid, block, 0, st, g.grid, g.d_ang, rays,
-                       g.P, g.S, ba_log2, xb, sino, out, maxw);
+    hipLaunchKernelGGL((k_forward<SB, kFwdY, E, LPR>), grid, block, 0, st, g.grid, g.d_ang,
+                       rays, g.P, g.S, ba_log2, xb, sino, out, maxw);
   else if (mode == kFwdSIRT)
-    hipLaunchKernelGGL((k_forward<SB, kFwdSIRT, E>), grid, block, 0, st, g.grid, g.d_ang,
+    hipLaunchKernelGGL((k_forward<SB, kFwdSIRT, E, LPR>), grid, block, 0, st, g.grid, g.d_ang,
                        rays, g.P, g.S, ba_log2, xb, sino, out, maxw);
   else
-    hipLaunchKernelGGL((k_forward<SB, kFwdEM, E>), grid, block, 0, st, g.grid, g.d_ang, rays,
-                       g.P, g.S, ba_log2, xb, sino, out, maxw);
+    hipLaunchKernelGGL((k_forward<SB, kFwdEM, E, LPR>), grid, block, 0, st, g.grid, g.d_ang,
+                       rays, g.P, g.S, ba_log2, xb, sino, out, maxw);
   return hipGetLastError();
+}
+
+template <int SB, int LPR>
+hipError_t fwd_exact(bool exact, int mode, const Geometry &g, const RayInfo *rays, int ba_log2,
+                     const float *xb, const float *sino, float *out, unsigned *maxw,
+                     hipStream_t st) {
+  return exact ? fwd_mode<SB, true, LPR>(mode, g, rays, ba_log2, xb, sino, out, maxw, st)
+               : fwd_mode<SB, false, LPR>(mode, g, rays, ba_log2, xb, sino, out, maxw, st);
 }
 
 constexpr size_t kLdsBytes = 160 * 1024;
@@ -872,7 +915,7 @@ template <int TW, int TH, int SB, int MODE, bool E>
 hipError_t bp_launch(TileArgs a, int nbatch, hipStream_t st) {
   const int N = a.g.N;
   const int tiles = ((N + TW - 1) / TW) * ((N + TH - 1) / TH);
-  size_t lds = (size_t)SB * TilePitch<TW, TH>::kPlane * sizeof(unsigned long long);
+  size_t lds = (size_t)SB * TilePitch<TW, TH>::kPlane * sizeof(BpWord<MODE>);
   // stage the per-angle column counts in LDS when they fit
   a.count_lds = (lds + (size_t)a.P * sizeof(int) <= kLdsBytes / 2) ? 1 : 0;
   if (a.count_lds) lds += (size_t)a.P * sizeof(int);
@@ -916,33 +959,27 @@ hipError_t launch_ray_info(const Geometry &g, RayInfo *rays, hipStream_t st) {
 }
 
 hipError_t launch_forward(const Geometry &g, const RayInfo *rays, int sb, int bundle_angles,
-                          int mode, bool exact, const float *xb, const float *sino,
-                          float *out, unsigned *maxw, hipStream_t st) {
+                          int lanes_per_ray, int mode, bool exact, const float *xb,
+                          const float *sino, float *out, unsigned *maxw, hipStream_t st) {
   int ba_log2 = 0;
   while ((1 << ba_log2) < bundle_angles && ba_log2 < 6) ++ba_log2;
   if ((1 << ba_log2) != bundle_angles) return hipErrorInvalidValue;
-#define TRACE_FWD_CASE(SB_)                                                            \
-  case SB_:                                                                            \
-    return exact ? fwd_mode<SB_, true>(mode, g, rays, ba_log2, xb, sino, out, maxw, st) \
-                 : fwd_mode<SB_, false>(mode, g, rays, ba_log2, xb, sino, out, maxw, st);
-  switch (sb) {
-    TRACE_FWD_CASE(1)
-    TRACE_FWD_CASE(2)
-    TRACE_FWD_CASE(4)
-    TRACE_FWD_CASE(8)
-    TRACE_FWD_CASE(16)
-    TRACE_FWD_CASE(32)
-    default: return hipErrorInvalidValue;
-  }
+#define TRACE_FWD_CASE(SB_, LPR_)                                                    \
+  if (sb == SB_ && lanes_per_ray == LPR_)                                            \
+    return fwd_exact<SB_, LPR_>(exact, mode, g, rays, ba_log2, xb, sino, out, maxw, st);
+  TRACE_FWD_SHAPES(TRACE_FWD_CASE)
 #undef TRACE_FWD_CASE
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_backproject(const Geometry &g, const RayInfo *rays, const int *count,
                               const TileShape &ts, int mode, bool exact, const float *w,
-                              const unsigned *maxw, float maxlen, float *z, hipStream_t st) {
+                              const unsigned *maxw, float maxlen, float denmax, float *z,
+                              hipStream_t st) {
   TileArgs a;
   a.maxw = maxw;
   a.maxlen = maxlen;
+  a.denmax = denmax;
   a.g = g.grid;
   a.ang = g.d_ang;
   a.rays = rays;

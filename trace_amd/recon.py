@@ -137,6 +137,12 @@ class Projector:
         check(lib().trace_set_option(self._h, _capi.TRACE_OPT_FWD_BUNDLE, int(angles)))
         self._fwd_bundle = int(angles)
 
+    def set_fwd_lanes(self, lanes: int):
+        """Lanes per ray of the forward projector (1, 2, 4); the slice batch
+        must be compiled for it (2: 8/16/32, 4: 16/32)."""
+        check(lib().trace_set_option(self._h, _capi.TRACE_OPT_FWD_LANES, int(lanes)))
+        self._fwd_lanes = int(lanes)
+
     def launch_shape(self) -> tuple:
         """(forward slice batch, tile_w, tile_h, tile slice batch)."""
         v = [ct.c_int() for _ in range(4)]
