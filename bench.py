@@ -93,18 +93,36 @@ def cpu_baseline(N, P_full, threads, target_s=12.0):
     }
 
 
-def traffic_record(kernel, workload):
+def kernel_source_hash() -> str:
+    """sha256 over the HIP sources and headers of the library: counter
+    records apply only to the kernels they were measured on."""
+    import hashlib
+    d = os.path.join(ROOT, "trace_amd", "csrc")
+    h = hashlib.sha256()
+    for f in sorted(os.listdir(d)):
+        if f.endswith((".hip", ".h")):
+            h.update(f.encode())
+            with open(os.path.join(d, f), "rb") as fh:
+                h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def traffic_record(kernel, workload, shape=None):
     """Counter-derived HBM bytes per launch (FETCH_SIZE and WRITE_SIZE
     scaled by a copy-kernel calibration) committed under profiles/, or None
-    if no record matches this kernel and workload."""
+    unless a record matches this kernel, workload, launch shape and kernel
+    sources."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(p) as f:
             recs = json.load(f)["records"]
     except Exception:
         return None
+    src = kernel_source_hash()
     for r in recs:
-        if r.get("kernel") == kernel and r.get("workload") == workload:
+        if (r.get("kernel") == kernel and r.get("workload") == workload
+                and r.get("source_sha") == src
+                and (shape is None or r.get("shape") == list(shape))):
             return r.get("bytes_per_launch")
     return None
 
@@ -125,6 +143,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default launch shape")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--shape", default="",
+                    help="fwd_batch,tile_w,tile_h,tile_batch,lanes,bundle: use this launch "
+                         "shape instead of autotuning (stack, pml, apmlr)")
     ap.add_argument("--workload", default="stack",
                     choices=["stack", "pml", "apmlr", "autograd", "batched"],
                     help="stack: the headline SIRT/MLEM stack (default); pml/apmlr: "
@@ -155,6 +176,16 @@ def main():
 def apply_shape(pj, tune):
     from trace_amd.autotune import apply_shape as _apply
     _apply(pj, tune)
+
+
+def forced_shape(args):
+    """--shape fb,tw,th,tb,lanes,bundle as an autotuner result, or None."""
+    if not args.shape:
+        return None
+    v = [int(x) for x in args.shape.split(",")]
+    if len(v) != 6:
+        raise SystemExit("--shape takes fwd_batch,tile_w,tile_h,tile_batch,lanes,bundle")
+    return {"shape": v[:4], "lanes": v[4], "bundle": v[5], "forced": True}
 
 
 def tune_shape(theta, sino, N, S, alg, fast_sqrt, params=None, init=None):
@@ -199,8 +230,11 @@ def run_stack(args):
     pj = Projector(theta, S, N, center=N / 2.0, device=0, exact_sqrt=not args.fast_sqrt)
     nnz, oor = pj.count_segments()
     log(f"segments per slice: {nnz} (outside image: {oor})")
-    tune = None
-    if not args.no_autotune:
+    tune = forced_shape(args)
+    if tune is not None:
+        apply_shape(pj, tune)
+        log(f"launch shape {tune['shape']} lanes {tune['lanes']} bundle {tune['bundle']} (given)")
+    elif not args.no_autotune:
         t0 = time.perf_counter()
         tune = tune_shape(theta, sino, N, S, args.alg, args.fast_sqrt)
         apply_shape(pj, tune)
@@ -234,6 +268,9 @@ def run_stack(args):
         "back": ("k_bp_tile", S * (8.0 * nnz + 4.0 * R + 4.0 * N * N)),
         "update": ("k_update_em", S * 16.0 * N * N),
     }
+    fb, tw, th, tb = pj.launch_shape()
+    lanes, bundle = pj.fwd_lanes(), pj.fwd_bundle()
+    shapes = {"forward": [fb, lanes, bundle], "back": [tw, th, tb], "update": [fb]}
     per_kernel = {}
     for ph, (kname, nbytes) in model.items():
         ms, n = phases[ph]
@@ -241,13 +278,14 @@ def run_stack(args):
         per_kernel[ph] = {"kernel": kname, "avg_launch_ms": t * 1e3, "launches": n,
                           "algorithmic_bytes_per_launch": nbytes,
                           "achieved_GBs": nbytes / t / 1e9 if t > 0 else None,
-                          "traffic_bytes_per_launch": traffic_record(kname, workload)}
+                          "traffic_bytes_per_launch": traffic_record(kname, workload,
+                                                                     shapes[ph])}
     dom = max(("forward", "back"), key=lambda k: per_kernel[k]["avg_launch_ms"])
     kernel = per_kernel[dom]["kernel"]
     t_dom = per_kernel[dom]["avg_launch_ms"] / 1e3
     bytes_dom = per_kernel[dom]["algorithmic_bytes_per_launch"]
     achieved = bytes_dom / t_dom / 1e9
-    traffic = traffic_record(kernel, workload)
+    traffic = per_kernel[dom]["traffic_bytes_per_launch"]
     b_alg_step = S * (16.0 * nnz + 12.0 * R + 24.0 * N * N)
 
     cpu = None
@@ -298,9 +336,9 @@ def run_stack(args):
         },
         "kernels": per_kernel,
         "launch_shape": {"fwd_batch_tile_w_h_batch": list(pj.launch_shape()),
-                         "fwd_bundle_angles": tune["bundle"] if tune else 8,
-                         "fwd_lanes_per_ray": tune.get("lanes", 1) if tune else 1,
-                         "autotuned": tune is not None},
+                         "fwd_bundle_angles": pj.fwd_bundle(),
+                         "fwd_lanes_per_ray": pj.fwd_lanes(),
+                         "autotuned": tune is not None and not tune.get("forced")},
         "phases_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in phases.items()},
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
@@ -335,8 +373,10 @@ def run_regularised(args):
     recon = torch.ones((S, N, N), dtype=torch.float32, device=dev)
     pj = Projector(theta, S, N, center=N / 2.0, device=0, exact_sqrt=not args.fast_sqrt)
     nnz, _ = pj.count_segments()
-    tune = None
-    if not args.no_autotune:
+    tune = forced_shape(args)
+    if tune is not None:
+        apply_shape(pj, tune)
+    elif not args.no_autotune:
         tune = tune_shape(theta, sino, N, S, alg, args.fast_sqrt, prm, init=1.0)
         apply_shape(pj, tune)
         log(f"autotuned shape {tune['shape']} bundle {tune['bundle']}")
@@ -382,7 +422,8 @@ def run_regularised(args):
         "roofline": {"bound": "hbm", "kernel": f"k_update_{alg}",
                      "achieved": upd_bytes / t_upd / 1e9, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": upd_bytes / t_upd / 1e9 / HBM_PEAK_GBS,
-                     "traffic": traffic_record(f"k_update_{alg}", workload),
+                     "traffic": traffic_record(f"k_update_{alg}", workload,
+                                               [pj.launch_shape()[0]]),
                      "algorithmic_bytes_per_launch": upd_bytes,
                      "avg_launch_ms": t_upd * 1e3},
         "slice_iter_per_s": slice_iters / elapsed,
@@ -390,9 +431,9 @@ def run_regularised(args):
         "stored_operator_frac": (S * (16.0 * nnz + 12.0 * R + 24.0 * N * N))
                                 / (elapsed / (iters * args.steps)) / 1e9 / HBM_PEAK_GBS,
         "launch_shape": {"fwd_batch_tile_w_h_batch": list(pj.launch_shape()),
-                         "fwd_bundle_angles": tune["bundle"] if tune else 8,
-                         "fwd_lanes_per_ray": tune.get("lanes", 1) if tune else 1,
-                         "autotuned": tune is not None},
+                         "fwd_bundle_angles": pj.fwd_bundle(),
+                         "fwd_lanes_per_ray": pj.fwd_lanes(),
+                         "autotuned": tune is not None and not tune.get("forced")},
         "checks": {"finite": bool(torch.isfinite(x).all().item()),
                    "nonnegative": bool((x >= 0).all().item())},
     }
@@ -514,9 +555,9 @@ def run_autograd(args):
                           "peak_MiB": peak / 2**20,
                           "grad_finite": bool(torch.isfinite(p.grad).all().item())},
         "launch_shape": {"fwd_batch_tile_w_h_batch": list(op.pj.launch_shape()),
-                         "fwd_bundle_angles": tune["bundle"] if tune else 8,
-                         "fwd_lanes_per_ray": tune.get("lanes", 1) if tune else 1,
-                         "autotuned": tune is not None},
+                         "fwd_bundle_angles": op.pj.fwd_bundle(),
+                         "fwd_lanes_per_ray": op.pj.fwd_lanes(),
+                         "autotuned": tune is not None and not tune.get("forced")},
     }
     print(json.dumps(out), flush=True)
 

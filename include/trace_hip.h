@@ -124,8 +124,11 @@ int trace_create(trace_handle_t **h, const trace_geom_t *g, int device,
 void trace_destroy(trace_handle_t *h);
 
 int trace_set_option(trace_handle_t *h, int option, int value);
+/* Current value of an option (TRACE_OPT_*). */
+int trace_get_option(trace_handle_t *h, int option, int *value);
 
-/* Launch shape: slices per forward-projector lane (1, 2, 4, 8, 16 or 32) and the
+/* Launch shape: slices per forward-projector ray (1, 2, 4, 8, 16 or 32; the
+ * lanes per ray are kept when the batch has that form, else reset to 1) and the
  * backprojector's image tile (tile_w x tile_h pixels, tile_batch slices per
  * workgroup).  Only the shapes compiled into the library are accepted
  * (TRACE_E_UNSUPPORTED otherwise); trace_amd.autotune picks among them.

@@ -102,6 +102,7 @@ _SIGS = {
     "trace_create": (ct.c_int, [ct.POINTER(_vp), ct.POINTER(TraceGeom), ct.c_int, _vp]),
     "trace_destroy": (None, [_vp]),
     "trace_set_option": (ct.c_int, [_vp, ct.c_int, ct.c_int]),
+    "trace_get_option": (ct.c_int, [_vp, ct.c_int, ct.POINTER(ct.c_int)]),
     "trace_set_launch_shape": (ct.c_int, [_vp, ct.c_int, ct.c_int, ct.c_int, ct.c_int]),
     "trace_get_launch_shape": (ct.c_int, [_vp, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int),
                                           ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),

@@ -48,9 +48,7 @@ LANES = (1, 2, 4)
 
 def apply_shape(pj, res: dict):
     """Sets {"shape": (fwd_batch, tile_w, tile_h, tile_batch), "bundle",
-    "lanes"} on projector `pj` (lanes first reset to 1 so that every
-    compiled batch is accepted)."""
-    pj.set_fwd_lanes(1)
+    "lanes"} on projector `pj`."""
     pj.set_fwd_bundle(1)
     pj.set_launch_shape(*res["shape"])
     pj.set_fwd_lanes(res.get("lanes", 1))

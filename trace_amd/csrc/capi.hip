@@ -280,6 +280,21 @@ int trace_create(trace_handle_t **out, const trace_geom_t *g, int device,
   auto *h = new trace_handle();
   h->device = device;
   h->stream = (hipStream_t)hip_stream;
+  // default launch shape by stack depth (the fastest measured at 2048^2,
+  // 1500 angles; trace_amd.autotune refines it for a given geometry)
+  const int S = g->num_slices;
+  if (S >= 32) {
+    h->fwd_sb = 32, h->fwd_lanes = 4, h->tile = TileShape{16, 16, 32};
+  } else if (S >= 16) {
+    h->fwd_sb = 16, h->fwd_lanes = 2, h->tile = TileShape{16, 16, 16};
+  } else if (S >= 8) {
+    h->fwd_sb = 8, h->fwd_lanes = 2, h->tile = TileShape{16, 16, 8};
+  } else if (S >= 3) {
+    h->fwd_sb = 4, h->fwd_lanes = 1, h->tile = TileShape{32, 32, 4};
+  } else {
+    h->fwd_sb = S, h->fwd_lanes = 1, h->tile = TileShape{32, 32, S};
+  }
+  h->fwd_bundle = 8;
   Geometry &geo = h->geo;
   geo.P = g->num_projs;
   geo.S = g->num_slices;
@@ -376,6 +391,19 @@ int trace_set_option(trace_handle_t *h, int option, int value) {
   }
 }
 
+int trace_get_option(trace_handle_t *h, int option, int *value) {
+  clear_error();
+  if (!h || !value) return fail(TRACE_E_INVALID, "null argument");
+  switch (option) {
+    case TRACE_OPT_EXACT_SQRT: *value = h->exact ? 1 : 0; return TRACE_OK;
+    case TRACE_OPT_PHASE_TIMING: *value = h->timer.on ? 1 : 0; return TRACE_OK;
+    case TRACE_OPT_BACKPROJECTOR: *value = h->backprojector; return TRACE_OK;
+    case TRACE_OPT_FWD_BUNDLE: *value = h->fwd_bundle; return TRACE_OK;
+    case TRACE_OPT_FWD_LANES: *value = h->fwd_lanes; return TRACE_OK;
+    default: return fail(TRACE_E_UNSUPPORTED, "unknown option");
+  }
+}
+
 int trace_set_launch_shape(trace_handle_t *h, int fwd_batch, int tile_w, int tile_h,
                            int tile_batch) {
   g_err.clear();
@@ -383,12 +411,12 @@ int trace_set_launch_shape(trace_handle_t *h, int fwd_batch, int tile_w, int til
   if (fwd_batch != 1 && fwd_batch != 2 && fwd_batch != 4 && fwd_batch != 8 &&
       fwd_batch != 16 && fwd_batch != 32)
     return fail(TRACE_E_UNSUPPORTED, "forward slice batch must be 1, 2, 4, 8, 16 or 32");
-  if (!fwd_shape_supported(fwd_batch, h->fwd_lanes))
-    return fail(TRACE_E_UNSUPPORTED,
-                "forward slice batch not compiled for the handle's lanes per ray");
+  if (!fwd_shape_supported(fwd_batch, 1))
+    return fail(TRACE_E_UNSUPPORTED, "forward slice batch not compiled");
   if (!tile_shape_supported(tile_w, tile_h, tile_batch))
     return fail(TRACE_E_UNSUPPORTED, "tile shape not compiled into this library");
   h->fwd_sb = fwd_batch;
+  if (!fwd_shape_supported(fwd_batch, h->fwd_lanes)) h->fwd_lanes = 1;
   h->tile = TileShape{tile_w, tile_h, tile_batch};
   return TRACE_OK;
 }

@@ -143,6 +143,17 @@ class Projector:
         check(lib().trace_set_option(self._h, _capi.TRACE_OPT_FWD_LANES, int(lanes)))
         self._fwd_lanes = int(lanes)
 
+    def get_option(self, option: int) -> int:
+        v = ct.c_int()
+        check(lib().trace_get_option(self._h, int(option), ct.byref(v)))
+        return v.value
+
+    def fwd_lanes(self) -> int:
+        return self.get_option(_capi.TRACE_OPT_FWD_LANES)
+
+    def fwd_bundle(self) -> int:
+        return self.get_option(_capi.TRACE_OPT_FWD_BUNDLE)
+
     def launch_shape(self) -> tuple:
         """(forward slice batch, tile_w, tile_h, tile slice batch)."""
         v = [ct.c_int() for _ in range(4)]
