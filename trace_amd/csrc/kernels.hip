@@ -537,6 +537,56 @@ __device__ __forceinline__ int pixel_class(int m, int n, int N) {
   return 4;
 }
 
+// In-plane neighbour offsets and PML weights of an inner pixel, in the
+// reference's q order (pml.cc:45-53, the first row of c_nb and c_pml_w),
+// as compile-time constants: inner pixels (all but the border) skip the
+// per-class tables.
+constexpr int kInnerDx[8] = {1, -1, 0, 0, 1, -1, 1, -1};
+constexpr int kInnerDy[8] = {0, 0, 1, -1, 1, 1, -1, -1};
+constexpr float kPmlAxial = 0.1464466094f;
+constexpr float kPmlDiag = 0.10355339059f;
+
+// PML stencil and update of the SB slices of batch b at one pixel.
+template <int SB, bool kInner>
+__device__ __forceinline__ void pml_pixel(const float *__restrict__ x, float *__restrict__ xn,
+                                          const float *__restrict__ z, int S, int N,
+                                          int64_t NN, int64_t pix, int cls, float d,
+                                          float beta, int b, float (&v)[SB]) {
+  const float *w = c_pml_w[cls == 0 ? 0 : (cls <= 4 ? 1 : 2)];
+  const int cnt = c_nbn[cls];
+#pragma unroll
+  for (int q = 0; q < SB; ++q) {
+    const int s = b * SB + q;
+    v[q] = 0.f;
+    if (s < S) {
+      const int64_t j = (int64_t)s * NN + pix;
+      const float x0 = x[j];
+      float f = 0.f, gg = 0.f;
+      if (kInner) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const float xq = x[j + kInnerDx[k] + (int64_t)kInnerDy[k] * N];
+          const float wk = k < 4 ? kPmlAxial : kPmlDiag;
+          f += 2.0f * beta * wk;
+          gg -= 2.0f * beta * wk * (x0 + xq);
+        }
+      } else {
+        for (int k = 0; k < cnt; ++k) {
+          const float xq = x[j + c_nb[cls][k][0] + (int64_t)c_nb[cls][k][1] * N];
+          f += 2.0f * beta * w[k];
+          gg -= 2.0f * beta * w[k] * (x0 + xq);
+        }
+      }
+      const float G = gg + d;
+      const float F = f;
+      const float nm = -(x0 * z[j]);
+      const float xv = (-G + __builtin_sqrtf(G * G - 8.0f * nm * F)) / (4.0f * F);
+      xn[j] = xv;
+      v[q] = xv;
+    }
+  }
+}
+
 // Fused PML stencil (pml.cc:25-201, F and G from zero) and closed-form
 // update (pml.cc:3-23):  x' = (-G + sqrt(G^2 - 8 num F)) / (4F), with
 // num = -x * (A^T w) (the x factor of pml.cc:243 taken out of the sum).
@@ -555,31 +605,12 @@ k_update_pml(const float *__restrict__ x, float *__restrict__ xn,
   const int64_t NN = (int64_t)N * N;
   const int64_t pix = (int64_t)n * N + m;
   const int cls = pixel_class(m, n, N);
-  const float *w = c_pml_w[cls == 0 ? 0 : (cls <= 4 ? 1 : 2)];
-  const int cnt = c_nbn[cls];
   const float d = den[pix];
   float v[SB];
-#pragma unroll
-  for (int q = 0; q < SB; ++q) {
-    const int s = b * SB + q;
-    v[q] = 0.f;
-    if (s < S) {
-      const int64_t j = (int64_t)s * NN + pix;
-      const float x0 = x[j];
-      float f = 0.f, gg = 0.f;
-      for (int k = 0; k < cnt; ++k) {
-        const float xq = x[j + c_nb[cls][k][0] + (int64_t)c_nb[cls][k][1] * N];
-        f += 2.0f * beta * w[k];
-        gg -= 2.0f * beta * w[k] * (x0 + xq);
-      }
-      const float G = gg + d;
-      const float F = f;
-      const float nm = -(x0 * z[j]);
-      const float xv = (-G + __builtin_sqrtf(G * G - 8.0f * nm * F)) / (4.0f * F);
-      xn[j] = xv;
-      v[q] = xv;
-    }
-  }
+  if (cls == 0)
+    pml_pixel<SB, true>(x, xn, z, S, N, NN, pix, cls, d, beta, b, v);
+  else
+    pml_pixel<SB, false>(x, xn, z, S, N, NN, pix, cls, d, beta, b, v);
   store_slices<SB>(xb + ((int64_t)b * NN + pix) * SB, v);
 }
 
@@ -590,32 +621,18 @@ struct ApmlrArgs {
   int nslots;  // slots of the batched copy
 };
 
-// Fused APMLR stencil (apmlr.cc:919-982 with FGInner/FGTop/FGBottom) and
-// update (apmlr.cc:3-26) over ghost-padded images [S+2][N][N]; real slice s
-// lives at padded index s+1.  The ray pass of slice s used padded slice s
-// (compat, apmlr.cc:1082-1084) or s+1, which also supplies num's x factor.
-// Thread (pixel, slot batch b): slot t of the batched copy holds padded
-// slice t + off (off = 0 compat, 1 otherwise); real slices are updated,
-// ghosts (padded 0 and S+1, constant) are copied, slots past S+1 hold 0.
-// The grid covers every real slice even when its slot lies past nslots.
-template <int SB>
-__global__ void __launch_bounds__(kPixBlock)
-k_update_apmlr(const float *__restrict__ xp, float *__restrict__ xnp,
-               const float *__restrict__ z, const float *__restrict__ den,
-               int S, int N, ApmlrArgs a, float *__restrict__ xb) {
-  const int m = blockIdx.x * kPixBlock + threadIdx.x;
-  const int n = blockIdx.y;
-  const int b = blockIdx.z;
-  if (m >= N) return;
-  const int64_t NN = (int64_t)N * N;
-  const int64_t pix = (int64_t)n * N + m;
-  const int cls = pixel_class(m, n, N);
-  const int cnt = c_nbn[cls];
+// APMLR stencil and update of the slots of batch b at one pixel (see
+// k_update_apmlr).
+template <int SB, bool kInner>
+__device__ __forceinline__ void apmlr_pixel(const float *__restrict__ xp,
+                                            float *__restrict__ xnp,
+                                            const float *__restrict__ z, int S, int N,
+                                            int64_t NN, int64_t pix, int cls, float d,
+                                            const ApmlrArgs &a, int b, float (&v)[SB]) {
+  const int cnt = kInner ? 8 : c_nbn[cls];
   const int off = a.compat ? 0 : 1;
   const float tb = 2.0f * a.p.beta;
   const float tb1 = 2.0f * a.p.beta1;
-  const float d = den[pix];
-  float v[SB];
 #pragma unroll
   for (int q = 0; q < SB; ++q) {
     const int pi = b * SB + q + off;  // padded slice of slot b*SB+q
@@ -627,16 +644,28 @@ k_update_apmlr(const float *__restrict__ xp, float *__restrict__ xnp,
       const bool top = (s == 0), bottom = (s == S - 1 && S - 1 != 0);
       const int zmode = top ? 1 : (bottom ? 2 : 0);
       const int nz = zmode == 0 ? 2 : 1;
-      const float *w = a.w.w[zmode][cls];
+      const float *w = a.w.w[zmode][kInner ? 0 : cls];
       const float x0 = xp[jp];
       float f = 0.f, gg = 0.f;
-      for (int k = 0; k < cnt; ++k) {
-        const float xq = xp[jp + c_nb[cls][k][0] + (int64_t)c_nb[cls][k][1] * N];
-        const float mg = x0 + xq;
-        const float rg = x0 - xq;
-        const float gm = 1.0f / (1.0f + __builtin_fabsf(rg / a.p.delta));
-        f += tb * w[k] * gm;
-        gg -= tb * w[k] * gm * mg;
+      if (kInner) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const float xq = xp[jp + kInnerDx[k] + (int64_t)kInnerDy[k] * N];
+          const float mg = x0 + xq;
+          const float rg = x0 - xq;
+          const float gm = 1.0f / (1.0f + __builtin_fabsf(rg / a.p.delta));
+          f += tb * w[k] * gm;
+          gg -= tb * w[k] * gm * mg;
+        }
+      } else {
+        for (int k = 0; k < cnt; ++k) {
+          const float xq = xp[jp + c_nb[cls][k][0] + (int64_t)c_nb[cls][k][1] * N];
+          const float mg = x0 + xq;
+          const float rg = x0 - xq;
+          const float gm = 1.0f / (1.0f + __builtin_fabsf(rg / a.p.delta));
+          f += tb * w[k] * gm;
+          gg -= tb * w[k] * gm * mg;
+        }
       }
       for (int k = 0; k < nz; ++k) {
         const int64_t dz = (zmode == 2 || k == 1) ? -NN : NN;
@@ -658,6 +687,34 @@ k_update_apmlr(const float *__restrict__ xp, float *__restrict__ xnp,
     }
     v[q] = val;
   }
+}
+
+// Fused APMLR stencil (apmlr.cc:919-982 with FGInner/FGTop/FGBottom) and
+// update (apmlr.cc:3-26) over ghost-padded images [S+2][N][N]; real slice s
+// lives at padded index s+1.  The ray pass of slice s used padded slice s
+// (compat, apmlr.cc:1082-1084) or s+1, which also supplies num's x factor.
+// Thread (pixel, slot batch b): slot t of the batched copy holds padded
+// slice t + off (off = 0 compat, 1 otherwise); real slices are updated,
+// ghosts (padded 0 and S+1, constant) are copied, slots past S+1 hold 0.
+// The grid covers every real slice even when its slot lies past nslots.
+template <int SB>
+__global__ void __launch_bounds__(kPixBlock)
+k_update_apmlr(const float *__restrict__ xp, float *__restrict__ xnp,
+               const float *__restrict__ z, const float *__restrict__ den,
+               int S, int N, ApmlrArgs a, float *__restrict__ xb) {
+  const int m = blockIdx.x * kPixBlock + threadIdx.x;
+  const int n = blockIdx.y;
+  const int b = blockIdx.z;
+  if (m >= N) return;
+  const int64_t NN = (int64_t)N * N;
+  const int64_t pix = (int64_t)n * N + m;
+  const int cls = pixel_class(m, n, N);
+  const float d = den[pix];
+  float v[SB];
+  if (cls == 0)
+    apmlr_pixel<SB, true>(xp, xnp, z, S, N, NN, pix, cls, d, a, b, v);
+  else
+    apmlr_pixel<SB, false>(xp, xnp, z, S, N, NN, pix, cls, d, a, b, v);
   if (b * SB < a.nslots) store_slices<SB>(xb + ((int64_t)b * NN + pix) * SB, v);
 }
 
