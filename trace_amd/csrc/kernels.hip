@@ -277,6 +277,20 @@ __device__ __forceinline__ int fixed_exponent(float maxw, float maxlen, bool &fi
 template <int MODE>
 using BpWord = typename std::conditional<MODE == kBpSino, unsigned, unsigned long long>::type;
 
+// Packed weights accumulators (SB >= 2): slices 2k and 2k+1 share a 64-bit
+// word, planes [SB/2][TH][TW + 1] of 64-bit words, the even slice in the
+// low half.  When every slice of the batch is fixed point, one ds_add_u64
+// adds both terms: with a, b the rounded terms, the 64-bit value
+// 2^32 b + a (low half a, high half b + sign(a)) sums exactly (far from
+// overflow), the low half of the sum is sum(a) (|sum(a)| < 2^31) and
+// (sum - sum(a)) / 2^32 is sum(b).  LDS atomics cost per instruction more
+// than per byte on gfx950, so this halves the dominant cost.  Otherwise
+// each half is its own 32-bit slot (integer or float atomics).
+template <int MODE, int SB>
+struct BpPacked {
+  static constexpr bool value = MODE == kBpSino && SB >= 2 && SB % 2 == 0;
+};
+
 // Main loop of the tile backprojector: lanes take consecutive (angle,
 // column) pairs of the flattened per-angle column ranges and add their
 // segments near the tile into the LDS accumulators.  ALLFX: every slice of
@@ -335,6 +349,30 @@ __device__ __forceinline__ void bp_tile_loop(const TileArgs &a, BpWord<MODE> *ac
             } else if (MODE == kBpOnes) {
               // |t| < 2^30: truncating conversion, error < 2^-e
               atomicAdd(dst, (BpWord<MODE>)(long long)(int)(l * wv[0]));
+            } else if (BpPacked<MODE, SB>::value) {
+              // 32-bit index of the pixel's 64-bit word in pair plane 0
+              unsigned *d2 = reinterpret_cast<unsigned *>(acc) + 2 * (ly * PITCH + lx);
+#pragma unroll
+              for (int k2 = 0; k2 < SB / 2; ++k2) {
+                unsigned *dk = d2 + 2 * k2 * PLANE;
+                if (ALLFX) {
+                  const int ia = (int)__builtin_rintf(l * wv[2 * k2]);
+                  const int ib = (int)__builtin_rintf(l * wv[2 * k2 + 1]);
+                  const unsigned long long v =
+                      ((unsigned long long)(unsigned)(ib + (ia >> 31)) << 32) | (unsigned)ia;
+                  atomicAdd(reinterpret_cast<unsigned long long *>(dk), v);
+                } else {
+#pragma unroll
+                  for (int h = 0; h < 2; ++h) {
+                    const int q = 2 * k2 + h;
+                    const float t = l * wv[q];
+                    if (fx[q])
+                      atomicAdd(dk + h, (unsigned)(int)__builtin_rintf(t));
+                    else
+                      atomicAdd(reinterpret_cast<float *>(dk + h), t);
+                  }
+                }
+              }
             } else {
 #pragma unroll
               for (int q = 0; q < SB; ++q) {
@@ -398,6 +436,33 @@ __device__ __forceinline__ void bp_tile_body(const TileArgs &a, int tile, int b,
   __syncthreads();
   // coalesced write-out: plane by plane, row-contiguous
   const int w_ = tx1 - tx0, h_ = ty1 - ty0;
+  if (BpPacked<MODE, SB>::value) {
+    const unsigned long long *acc64 = reinterpret_cast<const unsigned long long *>(acc);
+#pragma unroll
+    for (int k2 = 0; k2 < SB / 2; ++k2) {
+      for (int i = threadIdx.x; i < w_ * h_; i += kTileBlock) {
+        const int ly = i / w_, lx = i - ly * w_;
+        const unsigned long long v = acc64[k2 * PLANE + ly * PITCH + lx];
+        const int lo = (int)(unsigned)v;
+        const int hi = (int)(((long long)v - (long long)lo) >> 32);
+        const int64_t o = (int64_t)(ty0 + ly) * N + tx0 + lx;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int q = 2 * k2 + h;
+          const int s = b * SB + q;
+          if (s >= a.S) break;
+          const unsigned half = h ? (unsigned)(v >> 32) : (unsigned)v;
+          float out;
+          if (!fx[q])
+            out = __uint_as_float(half);
+          else
+            out = (float)ldexp((double)(all_fixed ? (h ? hi : lo) : (int)half), -ex[q]);
+          a.z[(int64_t)s * N * N + o] = out;
+        }
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int q = 0; q < SB; ++q) {
     const int s = b * SB + q;
