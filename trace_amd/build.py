@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtrace_hip.so")
 SOURCES = ["kernels.hip", "capi.hip", "batch.hip"]
-HEADERS = ["traverse.h", "internal.h"]
+HEADERS = ["traverse.h", "internal.h", "handle.h"]
 
 
 def hipcc() -> str:
@@ -55,19 +55,25 @@ def _check_isa(cc: str, src: str, odir: str, verbose: bool) -> None:
 
 
 def build(verbose: bool = False) -> str:
-    """Compiles every HIP source into LIB (always recompiles)."""
-    objs = []
+    """Compiles every HIP source into LIB (always recompiles); the object
+    and assembly passes of all sources run in parallel."""
+    from concurrent.futures import ThreadPoolExecutor
     cc = hipcc()
     odir = os.path.join(HERE, "_build")
     os.makedirs(odir, exist_ok=True)
-    for src in SOURCES:
-        obj = os.path.join(odir, src + ".o")
-        cmd = [cc, *flags(), "-c", os.path.join(CSRC, src), "-o", obj]
+    objs = [os.path.join(odir, src + ".o") for src in SOURCES]
+
+    def compile_one(i):
+        cmd = [cc, *flags(), "-c", os.path.join(CSRC, SOURCES[i]), "-o", objs[i]]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(obj)
-        _check_isa(cc, src, odir, verbose)
+
+    with ThreadPoolExecutor(max_workers=2 * len(SOURCES)) as ex:
+        jobs = [ex.submit(compile_one, i) for i in range(len(SOURCES))]
+        jobs += [ex.submit(_check_isa, cc, src, odir, verbose) for src in SOURCES]
+        for j in jobs:
+            j.result()  # re-raises a failed compile or a refused instruction
     tmp = LIB + ".tmp"
     cmd = [cc, "-shared", "-fPIC", "--offload-arch=gfx950", *objs, "-o", tmp]
     if verbose:
