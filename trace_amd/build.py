@@ -80,7 +80,51 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_programs(verbose)
     return LIB
+
+
+# HDF5 I/O and the reconstruction programs.  The I/O library is plain C so
+# that it needs libhdf5 (serial 1.10, installed under /opt/conda) and no
+# C++ runtime: loading it never pulls the older libstdc++ of that prefix
+# into a process.  Its DT_RPATH names the HDF5 directory; the programs find
+# both libraries next to them through $ORIGIN.
+HDF5_PREFIX = os.environ.get("TRACE_HDF5_PREFIX", "/opt/conda")
+H5LIB = os.path.join(HERE, "libtrace_h5io.so")
+BIN = os.path.join(HERE, "bin")
+PROGRAMS = {"sirt": 0, "mlem": 1, "pml": 2, "apmlr": 3, "trace": 4}
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+
+def build_programs(verbose: bool = False) -> None:
+    from concurrent.futures import ThreadPoolExecutor
+
+    def run(cmd):
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+
+    tmp = H5LIB + ".tmp"
+    run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-Wall", "-I", INCLUDE,
+         "-I", os.path.join(HDF5_PREFIX, "include"), os.path.join(CSRC, "h5io.c"),
+         "-o", tmp, "-L", os.path.join(HDF5_PREFIX, "lib"), "-lhdf5",
+         "-Wl,--disable-new-dtags,-rpath," + os.path.join(HDF5_PREFIX, "lib")])
+    os.replace(tmp, H5LIB)
+    os.makedirs(BIN, exist_ok=True)
+
+    def program(name):
+        out = os.path.join(BIN, name)
+        run(["g++", "-O2", "-std=c++17", "-Wall", f"-DTRACE_CLI_APP={PROGRAMS[name]}",
+             "-D__HIP_PLATFORM_AMD__", "-I", INCLUDE, "-I", os.path.join(ROCM, "include"),
+             os.path.join(CSRC, "trace_cli.cpp"), "-o", out + ".tmp", "-L", HERE,
+             "-ltrace_hip", "-ltrace_h5io", "-L", os.path.join(ROCM, "lib"), "-lamdhip64",
+             "-Wl,-rpath,$ORIGIN/..," + "-rpath," + os.path.join(ROCM, "lib")])
+        os.replace(out + ".tmp", out)
+
+    with ThreadPoolExecutor(max_workers=len(PROGRAMS)) as ex:
+        for j in [ex.submit(program, n) for n in PROGRAMS]:
+            j.result()
 
 
 if __name__ == "__main__":
