@@ -208,3 +208,34 @@ def test_slice_independence(gpu, alg):
             pj.reconstruct(alg, iters, torch.from_numpy(sino[:, s:s + 1, :].copy()).cuda(), dx, rp)
             alone = dx.cpu().numpy()[0]
         assert np.array_equal(alone.view(np.int32), stack[s].view(np.int32)), s
+
+
+@pytest.mark.parametrize("alg", ["sirt", "mlem", "pml"])
+def test_default_centre_is_reproducible(gpu, oracle, alg):
+    """With the reference's default centre (center <= 0 selects C/2 + 1,
+    trace_data.h:97) the rays of one edge column miss the image: y = 0 and
+    a2 = 0 give them inf/NaN weights (as in the reference), which add
+    nothing to A^T w.  They must not set the slice's fixed-point scale, or
+    the slice falls back to float accumulation: the result then varies
+    from run to run.  Checks that such rays exist, that two runs agree bit
+    for bit, and parity with the oracle."""
+    import torch
+    from trace_amd import Projector, RegParams
+    from trace_amd.phantom import angles_rad, noisy_sinogram
+    N, P, S, iters = 64, 60, 3, 3
+    th = angles_rad(P)
+    sino = np.abs(noisy_sinogram(th, N, S, seed=5, I0=1e4, K=8).numpy())
+    init = np.full((S, N, N), 0.0 if alg == "sirt" else 1.0, np.float32)
+    rp = RegParams(beta=0.5) if alg == "pml" else None
+    outs = []
+    with Projector(th, S, N, 0.0) as pj:
+        cnt, _, _, _ = pj.dump_rays(np.tile(np.arange(P), 2), np.repeat([0, N - 1], P))
+        assert (np.asarray(cnt) == 0).any(), "no ray misses the image"
+        for _ in range(2):
+            dx = torch.from_numpy(init).cuda()
+            pj.reconstruct(alg, iters, torch.from_numpy(sino).cuda(), dx, rp)
+            outs.append(dx.cpu().numpy())
+    assert np.array_equal(outs[0].view(np.int32), outs[1].view(np.int32))
+    xo = oracle.reconstruct(alg, th, sino, init, 0.0, iters, Params(beta=0.5) if rp else None)
+    assert np.isfinite(outs[0]).all()
+    assert rel_l2(outs[0], xo) <= 1e-4

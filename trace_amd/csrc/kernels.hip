@@ -101,7 +101,8 @@ __device__ __forceinline__ void store_slices(float *__restrict__ p, const float 
 //   SIRT  w = (p - y)/a2, a2 = sum len^2    (sirt.cc:47-52)
 //   EM    w = p/y                           (mlem.cc:50, pml.cc:236)
 // to w[p][c][s] (slices innermost, read as a block by the backprojector) and
-// fold max |w| per slice into maxw (bits; zeroed beforehand).
+// fold max |w| over the rays with a segment in the image per slice into
+// maxw (bits; zeroed beforehand).
 // LPR lanes per ray (1, 2 or 4): the lanes of one ray repeat its walk and
 // each gathers and sums SB/LPR of the batch's slices, so a gather touches
 // LPR-fold fewer distinct cache lines per wavefront instruction (the
@@ -133,6 +134,7 @@ __device__ __forceinline__ void forward_bundle(const GridInfo &g, const AngleInf
 #pragma unroll
   for (int k = 0; k < SBL; ++k) acc[k] = 0.f;
   float a2 = 0.f;
+  bool hit = false;  // the ray has a segment in the image
   const int64_t t = (int64_t)p * C + c;
   if (valid) {
     const RayInfo r = rays[t];
@@ -143,6 +145,7 @@ __device__ __forceinline__ void forward_bundle(const GridInfo &g, const AngleInf
         load_slices<SBL>(xbb + (int64_t)idx * SB, v);
 #pragma unroll
         for (int k = 0; k < SBL; ++k) acc[k] += v[k] * l;
+        hit = true;
       }
       if (MODE == kFwdSIRT) a2 += l2;
     });
@@ -159,7 +162,11 @@ __device__ __forceinline__ void forward_bundle(const GridInfo &g, const AngleInf
       } else {
         const float wv = (MODE == kFwdSIRT) ? (sino[o] - acc[k]) / a2 : sino[o] / acc[k];
         out[t * S + s] = wv;
-        m = __float_as_uint(__builtin_fabsf(wv));
+        // A ray with no segment in the image (a shifted rotation centre
+        // leaves edge columns outside) gets y = 0 and a2 = 0, so an
+        // inf/NaN weight, as in the reference; it adds nothing to A^T w
+        // there or here, so it must not set the slice's scale either.
+        if (hit) m = __float_as_uint(__builtin_fabsf(wv));
       }
     }
     if (MODE != kFwdY) {
