@@ -24,8 +24,10 @@
  *   independent.
  * - Layouts are the reference's: sinogram [P][S][C] (trace_data.h:139-149,
  *   trace_h5io.cc:157-205), image [S][N][N] with pixel index ix + iy*N.
- *   N must equal C (the reference hard-wires num_grids = num_cols,
- *   sirt.cc:182 / sirt_main.cc:135).
+ *   N and C may differ.  The reference's Reduce hard-wires num_grids =
+ *   num_cols (sirt.cc:107-108, sirt_main.cc:135), while its TraceMetadata
+ *   and traversal functions take them separately (trace_data.h:53-109,
+ *   trace_utils.cc:63-215); N != C follows those functions.
  * - Theta is in radians.  The reference's per-application preprocessing
  *   (degree conversion for SIRT/MLEM/APMLR, |sinogram| for MLEM/PML/APMLR,
  *   initial image 0 for SIRT and 1 otherwise) is the caller's, exactly as in
@@ -65,7 +67,7 @@ typedef struct {
   int num_projs;       /* P: projections (angles) */
   int num_slices;      /* S: slices */
   int num_cols;        /* C: detector columns */
-  int num_grids;       /* N: image side, must equal C */
+  int num_grids;       /* N: image side (the reference's mains pass N = C) */
   float center;        /* rotation centre; <= 0 selects C/2+1 (trace_data.h:97) */
   const float *theta;  /* [P] projection angles in radians, host memory (copied) */
 } trace_geom_t;
@@ -218,13 +220,14 @@ typedef struct trace_batch trace_batch_t;
 
 typedef struct {
   int num_projs;        /* P */
-  int num_cols;         /* C; the image is C x C (N = C, sirt.cc:182) */
+  int num_cols;         /* C: detector columns */
   float center;         /* rotation axis; <= 0 -> C/2 + 1 (trace_data.h:97) */
   const float *theta;   /* [P] radians (host; copied at creation) */
   trace_alg_t alg;      /* TRACE_SIRT or TRACE_MLEM */
   int iters;            /* iterations of this slice */
   const float *sino;    /* device [P][C] */
-  float *recon;         /* device [C][C]: initial image in, result out */
+  float *recon;         /* device [N][N]: initial image in, result out */
+  int num_grids;        /* N: image side; 0 selects N = C (the mains' choice) */
 } trace_batch_slice_t;
 
 /* Builds the per-slice tables (synchronous) on `device`, ordered on

@@ -81,20 +81,20 @@ class Oracle:
         L.or_ray_points.argtypes = [ct.c_int, ct.c_int, ct.c_float, ct.c_float,
                                     ct.c_int, _fp, _fp]
         L.or_ray_points.restype = ct.c_int
-        L.or_forward.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_float, _fp,
+        L.or_forward.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_float, _fp,
                                  _fp, _fp]
         L.or_forward.restype = ct.c_int
-        L.or_adjoint.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_float, _fp,
+        L.or_adjoint.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_float, _fp,
                                  _fp, _dp]
         L.or_adjoint.restype = ct.c_int
-        L.or_count_segments.argtypes = [ct.c_int, ct.c_int, ct.c_float, _fp,
+        L.or_count_segments.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_float, _fp,
                                         ct.POINTER(ct.c_int64)]
         L.or_count_segments.restype = ct.c_int64
-        L.or_reconstruct.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_int,
+        L.or_reconstruct.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int,
                                      _fp, ct.c_float, ct.c_int, _fp, _fp,
                                      ct.POINTER(OrParams), _fp, _fp, _fp, _fp]
         L.or_reconstruct.restype = ct.c_int
-        L.or_reconstruct_f64.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_int,
+        L.or_reconstruct_f64.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int,
                                          _fp, ct.c_float, ct.c_int, _fp, _dp,
                                          ct.POINTER(OrParams)]
         L.or_reconstruct_f64.restype = ct.c_int
@@ -149,29 +149,34 @@ class Oracle:
         n = self.L.or_ray_points(N, C, center, float(theta), col, _ptr(px), _ptr(py))
         return px[:n].copy(), py[:n].copy()
 
-    def count_segments(self, P, C, center, theta):
+    def count_segments(self, P, C, center, theta, N=None):
         th = _f32(theta)
         oor = ct.c_int64(0)
-        n = self.L.or_count_segments(P, C, center, _ptr(th), ct.byref(oor))
+        n = self.L.or_count_segments(P, C, C if N is None else N, center, _ptr(th),
+                                     ct.byref(oor))
         return int(n), int(oor.value)
 
     # -- operators -----------------------------------------------------
-    def forward(self, theta, x, center):
+    def forward(self, theta, x, center, C=None):
+        """y [P][S][C] = A x for x [S][N][N]; C detector columns (default N)."""
         x = _f32(x)
         S, N, _ = x.shape
+        C = N if C is None else int(C)
         th = _f32(theta)
         P = th.size
-        y = np.zeros((P, S, N), np.float32)
-        rc = self.L.or_forward(P, S, N, center, _ptr(th), _ptr(x), _ptr(y))
+        y = np.zeros((P, S, C), np.float32)
+        rc = self.L.or_forward(P, S, C, N, center, _ptr(th), _ptr(x), _ptr(y))
         assert rc == 0
         return y
 
-    def adjoint(self, theta, r, center):
+    def adjoint(self, theta, r, center, N=None):
+        """z [S][N][N] = A^T r for r [P][S][C]; N image side (default C)."""
         r = _f32(r)
         P, S, C = r.shape
+        N = C if N is None else int(N)
         th = _f32(theta)
-        z = np.zeros((S, C, C), np.float64)
-        rc = self.L.or_adjoint(P, S, C, center, _ptr(th), _ptr(r), _ptr(z, _dp))
+        z = np.zeros((S, N, N), np.float64)
+        rc = self.L.or_adjoint(P, S, C, N, center, _ptr(th), _ptr(r), _ptr(z, _dp))
         assert rc == 0
         return z
 
@@ -185,15 +190,16 @@ class Oracle:
         P, S, C = sino.shape
         th = _f32(theta_rad)
         x = _f32(recon0).copy()
+        N = x.shape[-1]
         prm = (params or Params()).c()
         num = den = F = G = None
         if want_stages:
-            num = np.zeros((S, C, C), np.float32)
+            num = np.zeros((S, N, N), np.float32)
             den = np.zeros_like(num)
             if alg in (PML, APMLR):
                 F = np.zeros_like(num)
                 G = np.zeros_like(num)
-        rc = self.L.or_reconstruct(alg, P, S, C, _ptr(th), center, iters,
+        rc = self.L.or_reconstruct(alg, P, S, C, N, _ptr(th), center, iters,
                                    _ptr(sino), _ptr(x), ct.byref(prm), _ptr(num),
                                    _ptr(den), _ptr(F), _ptr(G))
         assert rc == 0
@@ -208,8 +214,9 @@ class Oracle:
         P, S, C = sino.shape
         th = _f32(theta_rad)
         x = np.ascontiguousarray(recon0, dtype=np.float64).copy()
+        N = x.shape[-1]
         prm = (params or Params()).c()
-        rc = self.L.or_reconstruct_f64(alg, P, S, C, _ptr(th), center, iters,
+        rc = self.L.or_reconstruct_f64(alg, P, S, C, N, _ptr(th), center, iters,
                                        _ptr(sino), _ptr(x, _dp), ct.byref(prm))
         assert rc == 0
         return x
@@ -225,6 +232,9 @@ class RefLib:
         L.ref_trace_ray.argtypes = [ct.c_int, ct.c_float, ct.c_float, ct.c_int,
                                     _ip, _fp, _fp]
         L.ref_trace_ray.restype = ct.c_int
+        L.ref_trace_ray_n.argtypes = [ct.c_int, ct.c_int, ct.c_float, ct.c_float, ct.c_int,
+                                      _ip, _fp, _fp]
+        L.ref_trace_ray_n.restype = ct.c_int
         L.ref_forward.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_float, _fp,
                                   _fp, _fp]
         L.ref_forward.restype = ct.c_int
@@ -249,6 +259,15 @@ class RefLib:
         l2 = np.zeros(2 * C + 2, np.float32)
         n = self.L.ref_trace_ray(C, center, float(theta), col, _ptr(idx, _ip),
                                  _ptr(ln), _ptr(l2))
+        return idx[:n].copy(), ln[:n].copy(), l2[:n].copy()
+
+    def trace_ray_n(self, N, C, center, theta, col):
+        """A ray of C detector columns on an N x N grid."""
+        idx = np.zeros(2 * N + 2, np.int32)
+        ln = np.zeros(2 * N + 2, np.float32)
+        l2 = np.zeros(2 * N + 2, np.float32)
+        n = self.L.ref_trace_ray_n(N, C, center, float(theta), col, _ptr(idx, _ip),
+                                   _ptr(ln), _ptr(l2))
         return idx[:n].copy(), ln[:n].copy(), l2[:n].copy()
 
     def forward(self, theta, x, center):

@@ -72,6 +72,20 @@ int ref_trace_ray(int C, float center, float theta_q, int col, int *idx,
   return trace(md, b, theta_q, col, idx, len, len2);
 }
 
+// Segments of ray (theta_q, col) with C detector columns on an N x N grid:
+// the reference's TraceMetadata and traversal functions take num_cols and
+// num_grids separately (trace_data.h:53-109, trace_utils.cc:63-215) although
+// its Reduce passes num_grids = num_cols (sirt.cc:107-108).  Buffers hold
+// 2N+2 entries.
+int ref_trace_ray_n(int N, int C, float center, float theta_q, int col, int *idx,
+                    float *len, float *len2) {
+  float *th = new float[1];
+  th[0] = theta_q;
+  TraceMetadata md(th, 0, 0, 0, 1, 1, 1, C, N, center);
+  RayBufs b(N);
+  return trace(md, b, theta_q, col, idx, len, len2);
+}
+
 // y[p][s][c] = CalculateSimdata over each ray of slice s of x [S][C][C].
 int ref_forward(int P, int S, int C, float center, const float *theta,
                 const float *x, float *y) {

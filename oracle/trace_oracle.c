@@ -22,6 +22,12 @@
  *                         against _ref, fixed against the invariants).
  *   or_reconstruct_f64    fp64 twin; pinned by agreement with the fp32 path to
  *                         fp32 rounding level and by the same invariants.
+ *
+ * Image side N and detector columns C are separate parameters.  The
+ * reference's Reduce sets num_grids = num_cols (sirt.cc:107-108); its
+ * traversal functions (trace_utils.cc) and TraceMetadata (trace_data.h:
+ * 53-109) take them separately, and the N != C traversal is pinned against
+ * those functions (oracle/ref/harness_trace.cc: ref_trace_ray_n).
  */
 #include <math.h>
 #include <stdint.h>
@@ -255,9 +261,8 @@ int or_ray_points(int N, int C, float center, float theta, int col, float *px,
 /* y[p][s][c] = sum_k x_s[idx_k] * len_k, k ascending, fp32
  * (CalculateSimdata, sirt.cc:4-19).  Segments whose index falls outside the
  * slice are skipped (the reference would read past the image there). */
-int or_forward(int P, int S, int C, float center, const float *theta,
+int or_forward(int P, int S, int C, int N, float center, const float *theta,
                const float *x, float *y) {
-  const int N = C;
   const int64_t NN = (int64_t)N * N;
   or_ws w;
   int *idx = malloc((2 * (size_t)N + 2) * sizeof(int));
@@ -285,9 +290,8 @@ int or_forward(int P, int S, int C, float center, const float *theta,
 
 /* z_s[j] = sum over rays of len * r[p][s][c] (the adjoint A^T r), rays in
  * sinogram order; fp64 accumulation so the result is order-insensitive. */
-int or_adjoint(int P, int S, int C, float center, const float *theta,
+int or_adjoint(int P, int S, int C, int N, float center, const float *theta,
                const float *r, double *z) {
-  const int N = C;
   const int64_t NN = (int64_t)N * N;
   or_ws w;
   int *idx = malloc((2 * (size_t)N + 2) * sizeof(int));
@@ -315,9 +319,8 @@ int or_adjoint(int P, int S, int C, float center, const float *theta,
 
 /* Total segment count sum_rays (len-1) for one slice; also the number of
  * segments whose pixel index is out of the image (reported separately). */
-int64_t or_count_segments(int P, int C, float center, const float *theta,
+int64_t or_count_segments(int P, int C, int N, float center, const float *theta,
                           int64_t *out_of_range) {
-  const int N = C;
   or_ws w;
   int *idx = malloc((2 * (size_t)N + 2) * sizeof(int));
   float *len = malloc((2 * (size_t)N + 2) * sizeof(float));
@@ -492,11 +495,10 @@ static void apmlr_fg(const float *xr, const float *den, float *F, float *G, int 
  * initial image on entry and the result on return.  Optional outputs of the
  * last iteration: num_out/den_out [S][N][N] (before the update), F_out/G_out
  * [S][N][N] (PML/APMLR). */
-int or_reconstruct(int alg, int P, int S, int C, const float *theta,
+int or_reconstruct(int alg, int P, int S, int C, int N, const float *theta,
                    float center, int iters, const float *sino, float *recon,
                    const or_params *prm, float *num_out, float *den_out,
                    float *F_out, float *G_out) {
-  const int N = C;
   const int64_t NN = (int64_t)N * N;
   const int ghost = (alg == OR_APMLR) ? 1 : 0;
   const int64_t SP = (int64_t)S + 2 * ghost;
@@ -611,10 +613,9 @@ done:
 /* Same algorithm and row order as or_reconstruct, with the system matrix
  * taken from the fp32 traversal (the reference's A) and every accumulation,
  * weight and update carried in double.  The recon is double in and out. */
-int or_reconstruct_f64(int alg, int P, int S, int C, const float *theta,
+int or_reconstruct_f64(int alg, int P, int S, int C, int N, const float *theta,
                        float center, int iters, const float *sino,
                        double *recon, const or_params *prm) {
-  const int N = C;
   const int64_t NN = (int64_t)N * N;
   const int ghost = (alg == OR_APMLR) ? 1 : 0;
   const int64_t SP = (int64_t)S + 2 * ghost;

@@ -30,7 +30,10 @@ def _oracle_counts(oracle, th, N, center):
 
 @pytest.mark.parametrize("N,center,kind", [
     (64, 32.0, "even"), (63, 31.5, "odd"), (96, 0.0, "default-centre"),
-    (128, 60.3, "off-centre")])
+    (128, 60.3, "off-centre"),
+    # rays on grid lines: vertical rays along the right edge put every
+    # segment in pixel 0 of the next row (ix == N, trace_utils.cc:177-185)
+    (64, 32.5, "on-lines"), (63, 31.0, "on-lines-odd"), (40, 19.5, "on-lines-small")])
 def test_segment_counts_exact(gpu, oracle, N, center, kind):
     from trace_amd import Projector
     rng = np.random.default_rng(N)

@@ -223,3 +223,47 @@ def test_f64_twin_agrees(oracle, alg):
     x32 = oracle.reconstruct(alg, th, sino, x0, N / 2, iters, prm)
     x64 = oracle.reconstruct_f64(alg, th, sino, x0, N / 2, iters, prm)
     assert rel_l2(x32, x64) < 2e-5
+
+
+@pytest.mark.parametrize("N,C", [(48, 32), (32, 48)])
+def test_bruteforce_geometry_image_side_differs(oracle, N, C):
+    """N != C: per-pixel lengths against exact fp64 geometry (the C columns'
+    lines q.(-sin, cos) = yi across an N x N grid), as for N = C."""
+    rng = np.random.default_rng(N + 7 * C)
+    mov = oracle.mov(C, C / 2)
+    checked = 0
+    for _ in range(200):
+        th = np.float32(rng.uniform(0, np.pi))
+        col = int(rng.integers(0, C))
+        yi = float(np.float32((1 - C) / 2.0 + col + mov))
+        ex, pts = _exact_lengths(N, th, yi)
+        corners = np.array([[-N / 2, -N / 2], [-N / 2, N / 2], [N / 2, -N / 2], [N / 2, N / 2]])
+        if not pts or min(np.min(np.linalg.norm(corners - p, axis=1)) for p in (pts[0], pts[-1])) < 0.3:
+            continue
+        idx, ln, _ = oracle.trace_ray(N, C, C / 2, th, col)
+        got = {}
+        for i, l in zip(idx, ln):
+            got[int(i)] = got.get(int(i), 0.0) + float(l)
+        keys = set(got) | set(ex)
+        assert max(abs(got.get(k, 0) - ex.get(k, 0)) for k in keys) < 0.3
+        assert abs(sum(got.values()) - sum(ex.values())) < 0.03 * sum(ex.values())
+        checked += 1
+    assert checked > 100
+
+
+@pytest.mark.parametrize("N,C", [(40, 28), (28, 40)])
+def test_adjoint_dot_product_image_side_differs(oracle, N, C):
+    rng = np.random.default_rng(N + C)
+    P, S = 25, 2
+    th = rng.uniform(0, np.pi, P).astype(np.float32)
+    x = rng.uniform(0, 1, (S, N, N)).astype(np.float32)
+    r = rng.normal(size=(P, S, C)).astype(np.float32)
+    lhs = np.sum(oracle.forward(th, x, C / 2, C=C).astype(np.float64) * r)
+    rhs = np.sum(x.astype(np.float64) * oracle.adjoint(th, r, C / 2, N=N))
+    assert abs(lhs - rhs) <= 1e-5 * abs(rhs)
+    # A 1 is each ray's total length
+    y = oracle.forward(th, np.ones((1, N, N), np.float32), C / 2, C=C)
+    for p in range(0, P, 5):
+        for c in range(C):
+            _, ln, _ = oracle.trace_ray(N, C, C / 2, th[p], c)
+            assert abs(y[p, 0, c] - ln.astype(np.float64).sum()) <= 1e-5 * max(1, ln.sum())

@@ -80,3 +80,20 @@ def test_threaded_reference_close(oracle, ref, alg):
     x1 = ref.run(alg, th_file, sino, N / 2, iters, 1, prm)
     x4 = ref.run(alg, th_file, sino, N / 2, iters, 4, prm)
     assert rel_l2(x4, x1) < 1e-4
+
+
+@pytest.mark.parametrize("N,C", [(16, 24), (24, 16), (63, 40), (40, 63), (128, 97)])
+def test_traversal_bitexact_image_side_differs(oracle, ref, N, C):
+    """Image side N != detector columns C: the oracle against the
+    reference's own TraceMetadata and traversal functions on such a grid."""
+    rng = np.random.default_rng(N * 1000 + C)
+    angles = np.concatenate([adversarial_angles(), rng.uniform(-8, 8, 20).astype(np.float32)])
+    for th in angles:
+        for ctr in (C / 2.0, -1.0, C / 2.0 - 0.37):
+            for col in sorted({0, 1, C // 2, C - 2, C - 1, int(rng.integers(0, C))}):
+                a = oracle.trace_ray(N, C, ctr, th, col)
+                b = ref.trace_ray_n(N, C, ctr, th, col)
+                assert a[0].size == b[0].size, (N, C, th, ctr, col)
+                assert np.array_equal(a[0], b[0])
+                assert np.array_equal(_bits(a[1]), _bits(b[1]))
+                assert np.array_equal(_bits(a[2]), _bits(b[2]))

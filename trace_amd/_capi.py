@@ -63,6 +63,7 @@ class TraceBatchSlice(ct.Structure):
         ("iters", ct.c_int),
         ("sino", ct.c_void_p),
         ("recon", ct.c_void_p),
+        ("num_grids", ct.c_int),
     ]
 
 

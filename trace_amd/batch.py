@@ -32,7 +32,8 @@ BATCH_BUNDLES = (1, 2, 4, 8, 16)
 class BatchSlice:
     theta: np.ndarray   # [P] radians
     sino: object        # torch float32 [P][C] on the GPU
-    recon: object       # torch float32 [C][C] on the GPU (initial image in, result out)
+    recon: object       # torch float32 [N][N] on the GPU (initial image in, result out);
+                        # N = C in the reference's programs
     alg: str = "mlem"   # "sirt" or "mlem"
     iters: int = 25
     center: float = 0.0  # <= 0: C/2 + 1 (the reference's default)
@@ -50,12 +51,14 @@ class Batch:
         arr = (TraceBatchSlice * len(self.slices))()
         for i, s in enumerate(self.slices):
             P, C = s.sino.shape
-            if tuple(s.recon.shape) != (C, C):
-                raise ValueError(f"slice {i}: recon must be [{C}][{C}]")
+            N = int(s.recon.shape[0])
+            if s.recon.dim() != 2 or tuple(s.recon.shape) != (N, N):
+                raise ValueError(f"slice {i}: recon must be a square [N][N] image")
             if self._theta[i].size != P:
                 raise ValueError(f"slice {i}: theta length differs from the projections")
             arr[i].num_projs = P
             arr[i].num_cols = C
+            arr[i].num_grids = N
             arr[i].center = float(s.center)
             arr[i].theta = self._theta[i].ctypes.data_as(ct.POINTER(ct.c_float))
             arr[i].alg = _capi.ALGS[s.alg] if isinstance(s.alg, str) else int(s.alg)

@@ -70,9 +70,12 @@ void drop_graphs(trace_batch *b) {
   b->graphs.clear();
 }
 
+// Image side of a slice: num_grids, or C when it is 0.
+int slice_grids(const trace_batch_slice_t &s) { return s.num_grids > 0 ? s.num_grids : s.num_cols; }
+
 // Work items of one slice per phase for the batch's launch shape.
 void slice_items(const trace_batch *b, int i, int64_t out[3]) {
-  const int P = b->s[i].num_projs, C = b->s[i].num_cols, N = C;
+  const int P = b->s[i].num_projs, C = b->s[i].num_cols, N = slice_grids(b->s[i]);
   const int ba = b->bundle, bc = 64 / ba;
   const int64_t bundles = (int64_t)((P + ba - 1) / ba) * ((C + bc - 1) / bc);
   out[0] = (bundles + 3) / 4;  // four wavefront bundles per 256-lane workgroup
@@ -144,6 +147,7 @@ int prepare(trace_batch *b) {
     d.z = h->z;
     d.den = h->den;
     d.maxw = b->maxw + i;
+    d.wraps = h->wraps ? 1 : 0;
   }
   HIP_TRY(hipMemcpyAsync(b->d_desc, desc.data(), desc.size() * sizeof(BatchSliceDev),
                          hipMemcpyHostToDevice, b->user));
@@ -244,6 +248,7 @@ int trace_batch_create(trace_batch_t **out, const trace_batch_slice_t *slices, i
     if (s.alg != TRACE_SIRT && s.alg != TRACE_MLEM)
       return fail(TRACE_E_UNSUPPORTED, "batched slices run SIRT or MLEM");
     if (s.iters < 0) return fail(TRACE_E_INVALID, "iters must be >= 0");
+    if (s.num_grids < 0) return fail(TRACE_E_INVALID, "num_grids must be >= 0");
     if (!s.sino || !s.recon || !s.theta) return fail(TRACE_E_INVALID, "null slice buffer");
   }
   HIP_TRY(hipSetDevice(device < 0 ? 0 : device));
@@ -264,7 +269,7 @@ int trace_batch_create(trace_batch_t **out, const trace_batch_slice_t *slices, i
   b->nnz.resize(n);
   for (int i = 0; i < n; ++i) {
     const trace_batch_slice_t &s = slices[i];
-    trace_geom_t g{s.num_projs, 1, s.num_cols, s.num_cols, s.center, s.theta};
+    trace_geom_t g{s.num_projs, 1, s.num_cols, slice_grids(s), s.center, s.theta};
     trace_handle_t *h = nullptr;
     if (int rc = trace_create(&h, &g, device, hip_stream)) return bail(rc);
     b->h.push_back(h);

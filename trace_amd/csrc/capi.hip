@@ -46,15 +46,16 @@ int ensure_rays(trace_handle *h) {
   if (h->rays_ready) return TRACE_OK;
   if (!h->rays) HIP_TRY(dev_alloc(&h->rays, (size_t)h->geo.rays_per_slice()));
   TIMED(TRACE_PHASE_SETUP, launch_ray_info(h->geo, h->rays, h->stream));
-  HIP_TRY(hipMemsetAsync(h->counters, 0, 3 * sizeof(unsigned long long), h->stream));
+  HIP_TRY(hipMemsetAsync(h->counters, 0, 4 * sizeof(unsigned long long), h->stream));
   TIMED(TRACE_PHASE_SETUP, launch_count_segments(h->geo, h->rays, h->counters, h->counters + 1,
                                                  reinterpret_cast<unsigned *>(h->counters + 2),
-                                                 h->stream));
-  unsigned long long c[3] = {0, 0, 0};
+                                                 h->counters + 3, h->stream));
+  unsigned long long c[4] = {0, 0, 0, 0};
   HIP_TRY(hipMemcpyAsync(c, h->counters, sizeof(c), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
   h->nnz = (int64_t)c[0];
   h->nnz_oor = (int64_t)c[1];
+  h->wraps = c[3] > 0;
   const unsigned bits = (unsigned)c[2];
   std::memcpy(&h->maxlen, &bits, sizeof(float));
   h->rays_ready = true;
@@ -84,7 +85,7 @@ int backproject(trace_handle *h, int mode, const float *w, float *z, int phase) 
   if (h->backprojector == 0) {
     if (int rc = ensure_count(h)) return rc;
     TIMED(phase, launch_backproject(h->geo, h->rays, h->count, h->tile, mode, h->exact, w,
-                                    h->maxw, h->maxlen, h->den_max, z, h->stream));
+                                    h->maxw, h->maxlen, h->den_max, z, h->wraps, h->stream));
   } else {
     const int ns = (mode == kBpOutSino) ? h->geo.S : 1;
     TIMED(phase, launch_fill(z, (int64_t)ns * h->geo.NN(), 0.f, h->stream));
@@ -263,9 +264,8 @@ int trace_create(trace_handle_t **out, const trace_geom_t *g, int device,
   if (!g->theta) return fail(TRACE_E_INVALID, "theta is null");
   if (g->num_projs <= 0 || g->num_slices <= 0 || g->num_cols <= 1 || g->num_grids <= 1)
     return fail(TRACE_E_INVALID, "P, S must be >= 1 and C, N >= 2");
-  if (g->num_grids != g->num_cols)
-    return fail(TRACE_E_INVALID, "num_grids must equal num_cols (sirt.cc:182)");
-  if (g->num_cols > 32768) return fail(TRACE_E_INVALID, "num_cols above 32768");
+  if (g->num_cols > 32768 || g->num_grids > 32768)
+    return fail(TRACE_E_INVALID, "num_cols or num_grids above 32768");
   if (g->num_slices > 65535) return fail(TRACE_E_INVALID, "num_slices above 65535");
   if (!std::isfinite(g->center)) return fail(TRACE_E_INVALID, "center is not finite");
   if (device < 0) return fail(TRACE_E_INVALID, "device index is negative");
@@ -331,7 +331,7 @@ int trace_create(trace_handle_t **out, const trace_geom_t *g, int device,
       (e = dev_alloc(&geo.d_yi, (size_t)geo.C)) != hipSuccess ||
       (e = dev_alloc(&h->den, (size_t)geo.NN())) != hipSuccess ||
       (e = dev_alloc(&h->flag, 1)) != hipSuccess ||
-      (e = dev_alloc(&h->counters, 3)) != hipSuccess)
+      (e = dev_alloc(&h->counters, 4)) != hipSuccess)
     return cleanup_fail(fail(e == hipErrorOutOfMemory ? TRACE_E_NOMEM : TRACE_E_HIP,
                              std::string("allocation: ") + hipGetErrorString(e)));
   if ((e = hipMemcpy(geo.d_ang, h->ang.data(), h->ang.size() * sizeof(AngleInfo),

@@ -116,11 +116,13 @@ struct trace_handle {
   float *xa = nullptr;         // [S+2][N][N] work image (PML/APMLR)
   float *xc = nullptr;         // [S+2][N][N]
   int *flag = nullptr;
-  unsigned long long *counters = nullptr;  // [3]: segments, outside image, max length bits
+  unsigned long long *counters = nullptr;  // [4]: segments, outside image, max length bits,
+                                           // rays with a segment at ix == N
   unsigned *maxw = nullptr;    // [S] bits of max |w| per slice
   float maxlen = 0.f;          // longest segment of the geometry
   float den_max = 0.f;         // bound on the sum of segment lengths per pixel
   int64_t nnz = 0, nnz_oor = 0;
+  bool wraps = false;  // some ray has a segment with ix == N (the tile pass for it runs)
   trace_hip::PhaseTimer timer;
 };
 

@@ -75,6 +75,7 @@ struct BatchSliceDev {
   float *z;               // [N][N] A^T w
   const float *den;       // [N][N] A^T 1
   unsigned *maxw;         // bits of max |w| of this slice
+  int wraps;              // some ray has a segment with ix == N (kRayWraps)
 };
 
 // Work queues of one batch: phase f's items of slice order_f[k] are
@@ -109,7 +110,7 @@ struct BatchQueues {
 hipError_t launch_backproject(const Geometry &g, const RayInfo *rays, const int *count,
                               const TileShape &ts, int mode, bool exact, const float *w,
                               const unsigned *maxw, float maxlen, float denmax, float *z,
-                              hipStream_t st);
+                              bool wraps, hipStream_t st);
 hipError_t launch_absmax_slices(const float *w, int P, int S, int C, unsigned *maxw,
                                 hipStream_t st);
 hipError_t launch_to_pcs(const float *r, int P, int S, int C, float *w, unsigned *maxw,
@@ -146,7 +147,7 @@ hipError_t launch_ray_sums(const Geometry &g, const RayInfo *rays, bool exact, f
                            float *len2_sum, hipStream_t st);
 hipError_t launch_count_segments(const Geometry &g, const RayInfo *rays,
                                  unsigned long long *total, unsigned long long *oor,
-                                 unsigned *maxlen, hipStream_t st);
+                                 unsigned *maxlen, unsigned long long *wraps, hipStream_t st);
 hipError_t launch_check_finite(const float *v, int64_t n, int *flag, hipStream_t st);
 
 }  // namespace trace_hip

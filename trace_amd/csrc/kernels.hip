@@ -51,7 +51,9 @@ k_ray_info(GridInfo g, const AngleInfo *__restrict__ ang, const float *__restric
   const int64_t t = (int64_t)blockIdx.x * kRayBlock + threadIdx.x;
   if (t >= (int64_t)P * g.C) return;
   const int c = (int)(t % g.C), p = (int)(t / g.C);
-  rays[t] = make_ray(g, yi[c], ang[p].sinq, ang[p].cosq);
+  RayInfo r = make_ray(g, yi[c], ang[p].sinq, ang[p].cosq);
+  r.alo |= ray_flags(g, r, ang[p].quadrant);
+  rays[t] = r;
 }
 
 // ------------------------------------------------------------- forward --
@@ -225,6 +227,7 @@ struct TileArgs {
   float maxlen;       // longest segment of the geometry
   float denmax;       // bound on sum of lengths per pixel (max of A^T 1), kBpSino
   float *z;           // [S][N][N] output (kBpOnes/kBpCount: [N][N])
+  int wraps;          // some ray has a segment with ix == N (kRayWraps)
 };
 
 // Columns whose ray can touch tile [x0,x1]x[y0,y1] at angle p: the ray of
@@ -303,7 +306,19 @@ struct BpPacked {
 // segments near the tile into the LDS accumulators.  ALLFX: every slice of
 // the batch uses the fixed-point path (the common case; no per-slice
 // branch in the inner loop).
-template <int TW, int TH, int SB, int MODE, bool kExact, bool ALLFX>
+//
+// The reference's pixel index truncates midpoint + N/2 (trace_utils.cc:
+// 177-185), so a segment whose midpoint rounds onto the right image edge
+// gets ix == N, i.e. pixel 0 of the next row.  That happens to every
+// segment of a near-vertical ray lying within float rounding of the edge
+// line x = N/2 (rays on grid lines: an odd C - N, or a half-pixel shift of
+// the centre), and such a ray need not pass through the tile that holds
+// its target pixels.  pass 0 adds the segments whose midpoint is in the
+// tile; pass 1 (EDGE, tiles at column 0 of a geometry with such rays only)
+// walks the flagged rays through the strip just left of the right edge,
+// rows ty0-1 .. ty1-1, and adds only their wrapped segments.  Each segment
+// is added by exactly one pass.
+template <int TW, int TH, int SB, int MODE, bool kExact, bool ALLFX, bool EDGE>
 __device__ __forceinline__ void bp_tile_loop(const TileArgs &a, BpWord<MODE> *acc,
                                              const int *count, int tx0, int tx1, int ty0,
                                              int ty1, int b, const int (&ex)[SB],
@@ -312,11 +327,16 @@ __device__ __forceinline__ void bp_tile_loop(const TileArgs &a, BpWord<MODE> *ac
   constexpr int PLANE = TilePitch<TW, TH>::kPlane;
   const int N = a.g.N;
   const int C = a.g.C;
-  const float x0 = grid_line(N, tx0), x1 = grid_line(N, tx1);
-  const float y0 = grid_line(N, ty0), y1 = grid_line(N, ty1);
+  // region whose rays are walked, and its window in grid lines
+  const int wx0 = EDGE ? N - 1 : tx0, wx1 = EDGE ? N : tx1;
+  const int wy0 = EDGE ? ty0 - 1 : ty0, wy1 = ty1;
+  const float x0 = grid_line(N, wx0), x1 = grid_line(N, wx1);
+  const float y0 = grid_line(N, wy0), y1 = grid_line(N, wy1);
+  // the strip's column range is at most one wider than a tile's
+  constexpr int extra = EDGE ? 1 : 0;
   // lane t starts at flat pair t and advances by kTileBlock
   int p = 0, k = threadIdx.x;
-  while (p < a.P && k >= count[p]) { k -= count[p]; ++p; }
+  while (p < a.P && k >= count[p] + extra) { k -= count[p] + extra; ++p; }
   while (p < a.P) {
     int cmin, cmax;
     const AngleInfo ang = a.ang[p];
@@ -339,15 +359,18 @@ __device__ __forceinline__ void bp_tile_loop(const TileArgs &a, BpWord<MODE> *ac
       } else if (MODE == kBpOnes) {
         wv[0] = ldexpf(1.0f, ex[0]);
       }
-      if (any) {
-        const RayInfo r = a.rays[(int64_t)p * C + c];
-        traverse_window<kExact>(a.g, r, ang.quadrant, tx0, tx1, ty0, ty1,
+      const RayInfo r = a.rays[(int64_t)p * C + c];
+      if (any && (!EDGE || (r.alo & kRayWraps))) {
+        traverse_window<kExact>(a.g, r, ang.quadrant, wx0, wx1, wy0, wy1,
                                 [&](int ix, int iy, float l, float) {
-          // ix == N is pixel 0 of the next row (truncation)
+          // ix == N is pixel 0 of the next row (truncation); this pass takes
+          // the wrapped segments (EDGE) or the others (the bounds test below
+          // rejects the rest without a branch of its own)
           const bool wrap = (ix == N);
+          const bool take = EDGE ? wrap : !wrap;
           ix = wrap ? 0 : ix;
           iy = wrap ? iy + 1 : iy;
-          const unsigned lx = (unsigned)(ix - tx0);
+          const unsigned lx = take ? (unsigned)(ix - tx0) : 0xffffffffu;
           const unsigned ly = (unsigned)(iy - ty0);
           if (lx < (unsigned)(tx1 - tx0) && ly < (unsigned)(ty1 - ty0)) {
             BpWord<MODE> *dst = acc + ly * PITCH + lx;
@@ -396,7 +419,7 @@ __device__ __forceinline__ void bp_tile_loop(const TileArgs &a, BpWord<MODE> *ac
       }
     }
     k += kTileBlock;
-    while (p < a.P && k >= count[p]) { k -= count[p]; ++p; }
+    while (p < a.P && k >= count[p] + extra) { k -= count[p] + extra; ++p; }
   }
 }
 
@@ -436,10 +459,19 @@ __device__ __forceinline__ void bp_tile_body(const TileArgs &a, int tile, int b,
     all_fixed &= fx[q];
   }
   __syncthreads();
-  if (MODE == kBpCount || all_fixed)
-    bp_tile_loop<TW, TH, SB, MODE, kExact, true>(a, acc, count, tx0, tx1, ty0, ty1, b, ex, fx);
-  else
-    bp_tile_loop<TW, TH, SB, MODE, kExact, false>(a, acc, count, tx0, tx1, ty0, ty1, b, ex, fx);
+  if (MODE == kBpCount || all_fixed) {
+    bp_tile_loop<TW, TH, SB, MODE, kExact, true, false>(a, acc, count, tx0, tx1, ty0, ty1, b, ex,
+                                                        fx);
+    if (tx0 == 0 && a.wraps)
+      bp_tile_loop<TW, TH, SB, MODE, kExact, true, true>(a, acc, count, tx0, tx1, ty0, ty1, b,
+                                                         ex, fx);
+  } else {
+    bp_tile_loop<TW, TH, SB, MODE, kExact, false, false>(a, acc, count, tx0, tx1, ty0, ty1, b,
+                                                         ex, fx);
+    if (tx0 == 0 && a.wraps)
+      bp_tile_loop<TW, TH, SB, MODE, kExact, false, true>(a, acc, count, tx0, tx1, ty0, ty1, b,
+                                                          ex, fx);
+  }
   __syncthreads();
   // coalesced write-out: plane by plane, row-contiguous
   const int w_ = tx1 - tx0, h_ = ty1 - ty0;
@@ -869,6 +901,7 @@ k_bp_batch(const BatchSliceDev *__restrict__ d, BatchQueues q, int it) {
     a.maxlen = sd.maxlen;
     a.denmax = sd.denmax;
     a.z = sd.z;
+    a.wraps = sd.wraps;
     bp_tile_body<TW, TH, 1, kBpSino, kExact>(a, item - start[k], 0, acc);
   }
 }
@@ -971,15 +1004,18 @@ k_ray_sums(GridInfo g, const AngleInfo *__restrict__ ang, const RayInfo *__restr
 __global__ void __launch_bounds__(kRayBlock)
 k_count_segments(GridInfo g, const AngleInfo *__restrict__ ang,
                  const RayInfo *__restrict__ rays, int P,
-                 unsigned long long *total, unsigned long long *oor, unsigned *maxlen) {
+                 unsigned long long *total, unsigned long long *oor, unsigned *maxlen,
+                 unsigned long long *wraps) {
   const int64_t t = (int64_t)blockIdx.x * kRayBlock + threadIdx.x;
   const int C = g.C;
-  unsigned cnt = 0, bad = 0;
+  unsigned cnt = 0, bad = 0, wr = 0;
   float lmax = 0.f;
   if (t < (int64_t)P * C) {
     const int p = (int)(t / C);
     const unsigned NN = (unsigned)g.N * (unsigned)g.N;
-    cnt = traverse_full<true>(g, rays[t], ang[p].quadrant, [&](int ix, int iy, float l, float l2) {
+    const RayInfo r = rays[t];
+    wr = (r.alo & kRayWraps) ? 1u : 0u;
+    cnt = traverse_full<true>(g, r, ang[p].quadrant, [&](int ix, int iy, float l, float l2) {
       bad += ((unsigned)(ix + iy * g.N) >= NN) ? 1u : 0u;
       lmax = fmaxf(lmax, fmaxf(l, __builtin_amdgcn_sqrtf(l2)));
     });
@@ -988,11 +1024,13 @@ k_count_segments(GridInfo g, const AngleInfo *__restrict__ ang,
   for (int off = 32; off > 0; off >>= 1) {
     cnt += __shfl_down(cnt, off, 64);
     bad += __shfl_down(bad, off, 64);
+    wr += __shfl_down(wr, off, 64);
     lb = max(lb, (unsigned)__shfl_down((int)lb, off, 64));
   }
   if ((threadIdx.x & 63) == 0) {
     atomicAdd(total, (unsigned long long)cnt);
     if (bad) atomicAdd(oor, (unsigned long long)bad);
+    if (wr && wraps) atomicAdd(wraps, (unsigned long long)wr);
     atomicMax(maxlen, lb);
   }
 }
@@ -1104,8 +1142,9 @@ hipError_t launch_forward(const Geometry &g, const RayInfo *rays, int sb, int bu
 hipError_t launch_backproject(const Geometry &g, const RayInfo *rays, const int *count,
                               const TileShape &ts, int mode, bool exact, const float *w,
                               const unsigned *maxw, float maxlen, float denmax, float *z,
-                              hipStream_t st) {
+                              bool wraps, hipStream_t st) {
   TileArgs a;
+  a.wraps = wraps ? 1 : 0;
   a.maxw = maxw;
   a.maxlen = maxlen;
   a.denmax = denmax;
@@ -1340,10 +1379,10 @@ hipError_t launch_ray_sums(const Geometry &g, const RayInfo *rays, bool exact, f
 
 hipError_t launch_count_segments(const Geometry &g, const RayInfo *rays,
                                  unsigned long long *total, unsigned long long *oor,
-                                 unsigned *maxlen, hipStream_t st) {
+                                 unsigned *maxlen, unsigned long long *wraps, hipStream_t st) {
   const int64_t n = g.rays_per_slice();
   hipLaunchKernelGGL(k_count_segments, dim3(blocks_for(n, kRayBlock)), dim3(kRayBlock), 0, st,
-                     g.grid, g.d_ang, rays, g.P, total, oor, maxlen);
+                     g.grid, g.d_ang, rays, g.P, total, oor, maxlen, wraps);
   return hipGetLastError();
 }
 

@@ -54,11 +54,16 @@ struct GridInfo {
 
 // Per-ray constants: source point, slopes and the trimmed list ranges
 // (A: horizontal-line crossings n in [alo, alo+alen); B: vertical-line
-// crossings n in [blo, blo+blen)).  Computed once per geometry.
+// crossings n in [blo, blo+blen)).  Computed once per geometry.  Two flag
+// bits ride on alo (see ray_flags): kRayWhole, the tile walks must take the
+// whole ray; kRayWraps, the ray has a segment with ix == N.
 struct RayInfo {
   float srcx, srcy, slope, islope;
   int alo, alen, blo, blen;
 };
+
+constexpr int kRayWhole = 1 << 30;
+constexpr int kRayWraps = 1 << 29;
 
 // gridx[n] = gridy[n] = (float)(-N/2.0 + n) (trace_data.h:103-108), written
 // so that it is exact for both parities of N.
@@ -134,12 +139,14 @@ struct Walker {
   float srcx, srcy, slope, islope;
   int N, alo, alen, blo, blen, dA;
   bool fwd;
+  bool whole;           // windows are unsafe for this ray: walk it whole
   int i, j, nA, nB;     // A/B elements emitted so far; grid lines of the heads
   float ax, ay, bx, by; // heads (computed even past the end, never selected)
 
   __device__ __forceinline__ void init(const RayInfo &r, int N_, int quadrant) {
     srcx = r.srcx; srcy = r.srcy; slope = r.slope; islope = r.islope;
-    N = N_; alo = r.alo; alen = r.alen; blo = r.blo; blen = r.blen;
+    N = N_; alo = r.alo & ~(kRayWhole | kRayWraps); alen = r.alen; blo = r.blo; blen = r.blen;
+    whole = (r.alo & kRayWhole) != 0;
     fwd = quadrant != 0;  // A walked forward iff quadrant == 1 (:175,183,197)
     dA = fwd ? 1 : -1;
   }
@@ -247,6 +254,63 @@ __device__ __forceinline__ int traverse_full(const GridInfo &g, const RayInfo &r
   return walk<kExactSqrt>(w, g.mg, false, 0.f, 0.f, w.alen, w.blen, visit);
 }
 
+// Flags of a ray, from one walk of its merged points.
+//
+// kRayWraps: some segment's midpoint rounds onto the right image edge, so
+// its pixel index ix == N is pixel 0 of the next row (the reference's
+// truncation, trace_utils.cc:177-185).  Generic rays never do: their
+// crossings carry the 1/16-pixel granularity of the far source point.
+//
+// kRayWhole: a window walk of the ray would not be exact.  The window of a tile is a
+// range of grid lines along one axis (rows for |islope| <= 1, where the
+// A points carry exact row coordinates, columns otherwise).  The points of
+// the other list carry a computed coordinate along that axis; it must lie
+// within half a line of the span of the two exact points it is merged
+// between (1.5 lines before the first or after the last exact point), so
+// that every segment with its midpoint in the tile lies between exact
+// points inside the window and its margin.  Generic rays are off by less
+// than 0.2 line.  A ray lying within float rounding of a grid line (an odd
+// C - N, or a centre on a half pixel, at an axis-aligned angle) can have a
+// crossing of the parallel lines computed tens of pixels away; such rays are
+// walked whole by the tile kernels.
+__device__ __forceinline__ int ray_flags(const GridInfo &g, const RayInfo &r, int quadrant) {
+  Walker w;
+  w.init(r, g.N, quadrant);
+  const bool sorted = w.a_sorted();  // unsorted rays are walked whole in any case
+  w.seek(0, 0);
+  const bool on_a = __builtin_fabsf(r.islope) <= 1.0f;
+  bool have_exact = false, pending = false, safe = true, wraps = false;
+  float last = 0.f, cmin = 0.f, cmax = 0.f, px = 0.f;
+  const int n = w.alen + w.blen;
+  for (int k = 0; k < n; ++k) {
+    const bool takeA = (w.i < w.alen) && (w.j >= w.blen || w.ax < w.bx);
+    float qx, qy;
+    w.next(qx, qy);
+    if (k > 0) {
+      const float mx = (qx + px) * 0.5f;  // as segment()
+      wraps |= ((int)(mx + g.mg) == g.N);
+    }
+    px = qx;
+    const float v = on_a ? qy : qx;
+    if (takeA == on_a) {  // exact point: a grid line of the window axis
+      if (pending) {
+        const float lo = have_exact ? fminf(last, v) - 0.5f : v - 1.5f;
+        const float hi = have_exact ? fmaxf(last, v) + 0.5f : v + 1.5f;
+        safe &= (cmin >= lo && cmax <= hi);
+      }
+      have_exact = true;
+      pending = false;
+      last = v;
+    } else {
+      cmin = pending ? fminf(cmin, v) : v;
+      cmax = pending ? fmaxf(cmax, v) : v;
+      pending = true;
+    }
+  }
+  if (pending) safe &= have_exact ? (cmin >= last - 1.5f && cmax <= last + 1.5f) : (n < 2);
+  return (sorted && !safe ? kRayWhole : 0) | (wraps ? kRayWraps : 0);
+}
+
 // Rays whose merged points are used for a window of grid lines: every
 // segment whose midpoint falls in the lines' span [lo, hi] is produced,
 // together with a few neighbours (the callers filter by pixel).
@@ -269,7 +333,7 @@ __device__ __forceinline__ int traverse_window(const GridInfo &g, const RayInfo 
                                                Visit &&visit) {
   Walker w;
   w.init(r, g.N, quadrant);
-  if (!w.a_sorted()) {
+  if (!w.a_sorted() || w.whole) {
     w.seek(0, 0);
     return walk<kExactSqrt>(w, g.mg, false, 0.f, 0.f, w.alen, w.blen, visit);
   }
