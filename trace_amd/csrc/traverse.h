@@ -313,8 +313,13 @@ __device__ __forceinline__ int ray_flags(const GridInfo &g, const RayInfo &r, in
 
 // Rays whose merged points are used for a window of grid lines: every
 // segment whose midpoint falls in the lines' span [lo, hi] is produced,
-// together with a few neighbours (the callers filter by pixel).
-constexpr int kWindowMargin = 2;
+// together with a few neighbours (the callers filter by pixel).  One line
+// of margin suffices for the rays ray_flags passes: the points merged
+// between two consecutive exact points lie within half a line of their
+// span, so a segment with its midpoint in row r lies between exact points
+// on lines r-1 .. r+2 (1.5 lines before the first or after the last exact
+// point: within one line of margin as well).
+constexpr int kWindowMargin = 1;
 
 // Segments of the ray that can have their midpoint in pixel columns
 // [tx0, tx1) x rows [ty0, ty1): the ray is resumed just after the last A (or
