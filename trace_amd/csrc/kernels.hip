@@ -276,9 +276,12 @@ struct TilePitch {
 //   weights (kBpSino): 32-bit words, e per slice such that
 //     2^e * max|w| * denmax < 2^30, denmax >= the largest sum of segment
 //     lengths in one pixel (max of A^T 1); every pixel's sum then stays
-//     below 2^30 + (number of terms)/2, so it cannot overflow.  Terms are
-//     rounded to nearest (unbiased; error <= 2^-e/2 per term, about 2^-31
-//     of max|w| * denmax).
+//     below 2^30 + (number of terms)/2, so it cannot overflow; and
+//     2^e * max|w| * maxlen < 2^22, so that every term fits the rounding
+//     below.  A term is rounded to nearest even with one fma:
+//     fl(len * ws + 1.5 * 2^23) is 1.5 * 2^23 + rne(len * ws) exactly (floats
+//     in [2^23, 2^24) are the integers), so its bits minus those of
+//     1.5 * 2^23 are the integer term (unbiased; error <= 2^-e/2 per term).
 //   column sums (kBpOnes): 64-bit words, e from the longest segment
 //     (|term| < 2^30), truncated.
 // A slice whose weights are not all finite (y = 0 rays give inf/NaN in the
@@ -292,6 +295,29 @@ __device__ __forceinline__ int fixed_exponent(float maxw, float maxlen, bool &fi
   return 29 - ilogbf(m);
 }
 
+// Scale exponent of a weights slice (kBpSino): both bounds above.
+__device__ __forceinline__ int weight_exponent(float maxw, float denmax, float maxlen,
+                                               bool &fixed) {
+  int e = fixed_exponent(maxw, denmax, fixed);
+  const float t = maxw * maxlen;
+  fixed = fixed && __builtin_isfinite(t);
+  if (fixed && t != 0.f) e = min(e, 21 - ilogbf(t));
+  return fixed ? e : 0;
+}
+
+constexpr float kRoundMagic = 12582912.0f;        // 1.5 * 2^23
+constexpr unsigned kRoundMagicBits = 0x4B400000u;  // its bits
+
+// Bits of 1.5 * 2^23 + rne(len * ws), for |len * ws| < 2^22.
+__device__ __forceinline__ unsigned fixed_bits(float len, float ws) {
+  return __float_as_uint(__builtin_fmaf(len, ws, kRoundMagic));
+}
+
+// rne(len * ws) as an integer, for |len * ws| < 2^22.
+__device__ __forceinline__ int fixed_term(float len, float ws) {
+  return (int)(fixed_bits(len, ws) - kRoundMagicBits);
+}
+
 // Accumulator word of a mode: the weights backprojector sums 32-bit fixed
 // point (or float bits for a non-finite slice); column sums and counts use
 // 64-bit words.
@@ -300,13 +326,16 @@ using BpWord = typename std::conditional<MODE == kBpSino, unsigned, unsigned lon
 
 // Packed weights accumulators (SB >= 2): slices 2k and 2k+1 share a 64-bit
 // word, planes [SB/2][TH][TW + 1] of 64-bit words, the even slice in the
-// low half.  When every slice of the batch is fixed point, one ds_add_u64
-// adds both terms: with a, b the rounded terms, the 64-bit value
-// 2^32 b + a (low half a, high half b + sign(a)) sums exactly (far from
-// overflow), the low half of the sum is sum(a) (|sum(a)| < 2^31) and
-// (sum - sum(a)) / 2^32 is sum(b).  LDS atomics cost per instruction more
-// than per byte on gfx950, so this halves the dominant cost.  Otherwise
-// each half is its own 32-bit slot (integer or float atomics).
+// low half, then one plane [TH][TW + 1] of 32-bit per-pixel term counts.
+// When every slice of the batch is fixed point, one ds_add_u64 adds both
+// terms as the raw fma results: with a, b the rounded terms and M the bits
+// of 1.5 * 2^23, the 64-bit value 2^32 (M + b) + (M + a) is exact (both
+// halves are positive and below 2^31), so after n terms the word holds
+// 2^32 (n M + sum b) + n M + sum a (mod 2^64).  Subtracting n M (2^32 + 1)
+// leaves 2^32 sum(b) + sum(a): its low half is sum(a) (|sum(a)| < 2^31) and
+// the rest / 2^32 is sum(b).  A term then costs one fma per slice and the
+// two slices one ds_add_u64; the count costs one ds_add_u32 per segment.
+// Otherwise each half is its own 32-bit slot (integer or float atomics).
 template <int MODE, int SB>
 struct BpPacked {
   static constexpr bool value = MODE == kBpSino && SB >= 2 && SB % 2 == 0;
@@ -397,31 +426,31 @@ __device__ __forceinline__ void bp_tile_loop(const TileArgs &a, BpWord<MODE> *ac
               for (int k2 = 0; k2 < SB / 2; ++k2) {
                 unsigned *dk = d2 + 2 * k2 * PLANE;
                 if (ALLFX) {
-                  const int ia = (int)__builtin_rintf(l * wv[2 * k2]);
-                  const int ib = (int)__builtin_rintf(l * wv[2 * k2 + 1]);
                   const unsigned long long v =
-                      ((unsigned long long)(unsigned)(ib + (ia >> 31)) << 32) | (unsigned)ia;
+                      ((unsigned long long)fixed_bits(l, wv[2 * k2 + 1]) << 32) |
+                      fixed_bits(l, wv[2 * k2]);
                   atomicAdd(reinterpret_cast<unsigned long long *>(dk), v);
                 } else {
 #pragma unroll
                   for (int h = 0; h < 2; ++h) {
                     const int q = 2 * k2 + h;
-                    const float t = l * wv[q];
                     if (fx[q])
-                      atomicAdd(dk + h, (unsigned)(int)__builtin_rintf(t));
+                      atomicAdd(dk + h, (unsigned)fixed_term(l, wv[q]));
                     else
-                      atomicAdd(reinterpret_cast<float *>(dk + h), t);
+                      atomicAdd(reinterpret_cast<float *>(dk + h), l * wv[q]);
                   }
                 }
               }
+              // terms per pixel, for the magic offsets of the ALLFX words
+              if (ALLFX)
+                atomicAdd(reinterpret_cast<unsigned *>(acc) + SB * PLANE + ly * PITCH + lx, 1u);
             } else {
 #pragma unroll
               for (int q = 0; q < SB; ++q) {
-                const float t = l * wv[q];
                 if (ALLFX || fx[q]) {
-                  atomicAdd(dst + q * PLANE, (BpWord<MODE>)(int)__builtin_rintf(t));
+                  atomicAdd(dst + q * PLANE, (BpWord<MODE>)fixed_term(l, wv[q]));
                 } else {
-                  atomicAdd(reinterpret_cast<float *>(dst + q * PLANE), t);
+                  atomicAdd(reinterpret_cast<float *>(dst + q * PLANE), l * wv[q]);
                 }
               }
             }
@@ -434,9 +463,16 @@ __device__ __forceinline__ void bp_tile_loop(const TileArgs &a, BpWord<MODE> *ac
   }
 }
 
+// LDS words (BpWord<MODE>) of a tile's accumulators: the slice planes, and
+// for packed weights the per-pixel term counts.
+template <int TW, int TH, int SB, int MODE>
+constexpr int bp_acc_words() {
+  return (SB + (BpPacked<MODE, SB>::value ? 1 : 0)) * TilePitch<TW, TH>::kPlane;
+}
+
 // Workgroup (tile, slice batch): LDS accumulators acc[SB][TH][TW + 1] for
 // all angles, then one coalesced write-out.  `acc` is the workgroup's
-// dynamic LDS ([SB][TH][TW + 1] words of BpWord<MODE>, then int count[P]
+// dynamic LDS (bp_acc_words() words of BpWord<MODE>, then int count[P]
 // when a.count_lds); the body starts and ends with a workgroup barrier, so a
 // persistent workgroup may run it for one tile after another.
 template <int TW, int TH, int SB, int MODE, bool kExact>
@@ -445,7 +481,8 @@ __device__ __forceinline__ void bp_tile_body(const TileArgs &a, int tile, int b,
   BpWord<MODE> *acc = reinterpret_cast<BpWord<MODE> *>(lds);
   constexpr int PITCH = TilePitch<TW, TH>::kPitch;
   constexpr int PLANE = TilePitch<TW, TH>::kPlane;
-  int *lcount = reinterpret_cast<int *>(acc + SB * PLANE);
+  constexpr int WORDS = bp_acc_words<TW, TH, SB, MODE>();
+  int *lcount = reinterpret_cast<int *>(acc + WORDS);
   const int *count = a.count_lds ? lcount : a.count;
   const int N = a.g.N;
   const int tiles_x = (N + TW - 1) / TW;
@@ -453,7 +490,7 @@ __device__ __forceinline__ void bp_tile_body(const TileArgs &a, int tile, int b,
   const int tx0 = tx * TW, ty0 = ty * TH;
   const int tx1 = min(tx0 + TW, N), ty1 = min(ty0 + TH, N);
   __syncthreads();  // the previous tile's write-out has read acc
-  for (int i = threadIdx.x; i < SB * PLANE; i += kTileBlock) acc[i] = 0;
+  for (int i = threadIdx.x; i < WORDS; i += kTileBlock) acc[i] = 0;
   if (a.count_lds)
     for (int i = threadIdx.x; i < a.P; i += kTileBlock) lcount[i] = a.count[i];
   // per-slice scale exponents (uniform across the workgroup)
@@ -464,7 +501,8 @@ __device__ __forceinline__ void bp_tile_body(const TileArgs &a, int tile, int b,
   for (int q = 0; q < SB; ++q) {
     const int s = b * SB + q;
     if (MODE == kBpSino)
-      ex[q] = fixed_exponent((s < a.S) ? __uint_as_float(a.maxw[s]) : 0.f, a.denmax, fx[q]);
+      ex[q] = weight_exponent((s < a.S) ? __uint_as_float(a.maxw[s]) : 0.f, a.denmax, a.maxlen,
+                              fx[q]);
     else
       ex[q] = fixed_exponent(1.0f, a.maxlen, fx[q]);
     all_fixed &= fx[q];
@@ -488,11 +526,16 @@ __device__ __forceinline__ void bp_tile_body(const TileArgs &a, int tile, int b,
   const int w_ = tx1 - tx0, h_ = ty1 - ty0;
   if (BpPacked<MODE, SB>::value) {
     const unsigned long long *acc64 = reinterpret_cast<const unsigned long long *>(acc);
+    const unsigned *nterms = reinterpret_cast<const unsigned *>(acc) + SB * PLANE;
 #pragma unroll
     for (int k2 = 0; k2 < SB / 2; ++k2) {
       for (int i = threadIdx.x; i < w_ * h_; i += kTileBlock) {
         const int ly = i / w_, lx = i - ly * w_;
-        const unsigned long long v = acc64[k2 * PLANE + ly * PITCH + lx];
+        unsigned long long v = acc64[k2 * PLANE + ly * PITCH + lx];
+        // remove the magic offsets: n * (2^32 + 1) * bits(1.5 * 2^23), mod 2^64
+        if (all_fixed)
+          v -= (unsigned long long)nterms[ly * PITCH + lx] *
+               (((unsigned long long)kRoundMagicBits << 32) | kRoundMagicBits);
         const int lo = (int)(unsigned)v;
         const int hi = (int)(((long long)v - (long long)lo) >> 32);
         const int64_t o = (int64_t)(ty0 + ly) * N + tx0 + lx;
@@ -1093,7 +1136,7 @@ template <int TW, int TH, int SB, int MODE, bool E>
 hipError_t bp_launch(TileArgs a, int nbatch, hipStream_t st) {
   const int N = a.g.N;
   const int tiles = ((N + TW - 1) / TW) * ((N + TH - 1) / TH);
-  size_t lds = (size_t)SB * TilePitch<TW, TH>::kPlane * sizeof(BpWord<MODE>);
+  size_t lds = (size_t)bp_acc_words<TW, TH, SB, MODE>() * sizeof(BpWord<MODE>);
   // stage the per-angle column counts in LDS when they fit
   a.count_lds = (lds + (size_t)a.P * sizeof(int) <= kLdsBytes / 2) ? 1 : 0;
   if (a.count_lds) lds += (size_t)a.P * sizeof(int);
