@@ -101,11 +101,14 @@ typedef struct trace_handle trace_handle_t;
 #define TRACE_OPT_FWD_BUNDLE 4 /* adjacent angles per forward-projector
                                   wavefront (1, 2, 4, 8, 16; the 64 lanes are
                                   angles x 64/(angles x lanes) adjacent
-                                  columns x lanes per ray) */
+                                  columns x lanes per ray); default 16 at
+                                  four lanes per ray, else 8 */
 #define TRACE_OPT_FWD_LANES 5 /* lanes per ray of the forward projector
-                                 (1 default, 2, 4): the lanes of a ray split
-                                 its slice batch; compiled for batches 8, 16
-                                 and 32 (2 lanes) and 16, 32 (4 lanes) */
+                                 (1, 2, 4; default by stack depth: 4 at 32 or
+                                 more slices, 2 at 8-31, else 1): the lanes
+                                 of a ray split its slice batch; compiled for
+                                 batches 8, 16 and 32 (2 lanes) and 16, 32
+                                 (4 lanes) */
 
 /* Kernel phases reported by trace_phase_times. */
 #define TRACE_PHASE_FORWARD 0 /* forward projection and per-ray weights */
