@@ -64,6 +64,7 @@ struct RayInfo {
 
 constexpr int kRayWhole = 1 << 30;
 constexpr int kRayWraps = 1 << 29;
+constexpr int kRay1D = 1 << 28;
 
 // gridx[n] = gridy[n] = (float)(-N/2.0 + n) (trace_data.h:103-108), written
 // so that it is exact for both parities of N.
@@ -140,13 +141,15 @@ struct Walker {
   int N, alo, alen, blo, blen, dA;
   bool fwd;
   bool whole;           // windows are unsafe for this ray: walk it whole
+  bool one_axis;        // only the primary axis may bound its windows
   int i, j, nA, nB;     // A/B elements emitted so far; grid lines of the heads
   float ax, ay, bx, by; // heads (computed even past the end, never selected)
 
   __device__ __forceinline__ void init(const RayInfo &r, int N_, int quadrant) {
     srcx = r.srcx; srcy = r.srcy; slope = r.slope; islope = r.islope;
-    N = N_; alo = r.alo & ~(kRayWhole | kRayWraps); alen = r.alen; blo = r.blo; blen = r.blen;
+    N = N_; alo = r.alo & ~(kRayWhole | kRayWraps | kRay1D); alen = r.alen; blo = r.blo; blen = r.blen;
     whole = (r.alo & kRayWhole) != 0;
+    one_axis = (r.alo & kRay1D) != 0;
     fwd = quadrant != 0;  // A walked forward iff quadrant == 1 (:175,183,197)
     dA = fwd ? 1 : -1;
   }
@@ -273,26 +276,15 @@ __device__ __forceinline__ int traverse_full(const GridInfo &g, const RayInfo &r
 // C - N, or a centre on a half pixel, at an axis-aligned angle) can have a
 // crossing of the parallel lines computed tens of pixels away; such rays are
 // walked whole by the tile kernels.
-__device__ __forceinline__ int ray_flags(const GridInfo &g, const RayInfo &r, int quadrant) {
-  Walker w;
-  w.init(r, g.N, quadrant);
-  const bool sorted = w.a_sorted();  // unsorted rays are walked whole in any case
-  w.seek(0, 0);
-  const bool on_a = __builtin_fabsf(r.islope) <= 1.0f;
-  bool have_exact = false, pending = false, safe = true, wraps = false;
-  float last = 0.f, cmin = 0.f, cmax = 0.f, px = 0.f;
-  const int n = w.alen + w.blen;
-  for (int k = 0; k < n; ++k) {
-    const bool takeA = (w.i < w.alen) && (w.j >= w.blen || w.ax < w.bx);
-    float qx, qy;
-    w.next(qx, qy);
-    if (k > 0) {
-      const float mx = (qx + px) * 0.5f;  // as segment()
-      wraps |= ((int)(mx + g.mg) == g.N);
-    }
-    px = qx;
+// An axis check per axis: the window of the second axis may be intersected
+// with the first one's (kRay1D is set when the second axis fails).
+struct AxisCheck {
+  bool on_a, have_exact = false, pending = false, safe = true;
+  float last = 0.f, cmin = 0.f, cmax = 0.f;
+  __device__ __forceinline__ explicit AxisCheck(bool a) : on_a(a) {}
+  __device__ __forceinline__ void point(bool takeA, float qx, float qy) {
     const float v = on_a ? qy : qx;
-    if (takeA == on_a) {  // exact point: a grid line of the window axis
+    if (takeA == on_a) {  // exact point: a grid line of this axis
       if (pending) {
         const float lo = have_exact ? fminf(last, v) - 0.5f : v - 1.5f;
         const float hi = have_exact ? fmaxf(last, v) + 0.5f : v + 1.5f;
@@ -307,8 +299,37 @@ __device__ __forceinline__ int ray_flags(const GridInfo &g, const RayInfo &r, in
       pending = true;
     }
   }
-  if (pending) safe &= have_exact ? (cmin >= last - 1.5f && cmax <= last + 1.5f) : (n < 2);
-  return (sorted && !safe ? kRayWhole : 0) | (wraps ? kRayWraps : 0);
+  __device__ __forceinline__ bool done(int n) const {
+    if (!pending) return safe;
+    return safe && (have_exact ? (cmin >= last - 1.5f && cmax <= last + 1.5f) : (n < 2));
+  }
+};
+
+__device__ __forceinline__ int ray_flags(const GridInfo &g, const RayInfo &r, int quadrant) {
+  Walker w;
+  w.init(r, g.N, quadrant);
+  const bool sorted = w.a_sorted();  // unsorted rays are walked whole in any case
+  w.seek(0, 0);
+  const bool on_a = __builtin_fabsf(r.islope) <= 1.0f;
+  AxisCheck prim(on_a), sec(!on_a);
+  bool wraps = false;
+  float px = 0.f;
+  const int n = w.alen + w.blen;
+  for (int k = 0; k < n; ++k) {
+    const bool takeA = (w.i < w.alen) && (w.j >= w.blen || w.ax < w.bx);
+    float qx, qy;
+    w.next(qx, qy);
+    if (k > 0) {
+      const float mx = (qx + px) * 0.5f;  // as segment()
+      wraps |= ((int)(mx + g.mg) == g.N);
+    }
+    px = qx;
+    prim.point(takeA, qx, qy);
+    sec.point(takeA, qx, qy);
+  }
+  const bool whole = sorted && !prim.done(n);
+  const bool one_axis = !sec.done(n);
+  return (whole ? kRayWhole : 0) | (wraps ? kRayWraps : 0) | (one_axis ? kRay1D : 0);
 }
 
 // Rays whose merged points are used for a window of grid lines: every
@@ -322,17 +343,21 @@ __device__ __forceinline__ int ray_flags(const GridInfo &g, const RayInfo &r, in
 constexpr int kWindowMargin = 1;
 
 // Segments of the ray that can have their midpoint in pixel columns
-// [tx0, tx1) x rows [ty0, ty1): the ray is resumed just after the last A (or
-// B) element that lies kWindowMargin lines before the window and stopped
-// kWindowMargin lines after it.  Rays with |islope| <= 1 cross more
-// horizontal lines than vertical ones and are windowed on the A list (rows),
-// others on the B list (columns).  Consecutive merged points lie within
-// about one line of each other along the window axis (the crossing noise of
-// the reference's far source point is below 0.2 pixel), so the margin
-// covers every segment of the tile.  A rays whose A list is not ascending
-// (angles outside [0, pi) where the quadrant rule and the slope sign
-// disagree) are walked whole.
-template <bool kExactSqrt, typename Visit>
+// [tx0, tx1) x rows [ty0, ty1).  The window of the A list (rows) is the A
+// elements on lines [ty0 - M, ty1 + M], that of the B list (columns) the B
+// elements on lines [tx0 - M, tx1 + M].  The walk covers the merged points
+// from just after the later of the elements preceding the two windows to the
+// earlier of their last elements (kTwoAxes), or of the primary window alone
+// (rows for |islope| <= 1, where the A points carry exact row coordinates;
+// columns otherwise) without kTwoAxes and for rays flagged kRay1D.  Two axes
+// walk about 30% fewer segments, but the rays of a tile then differ in walk
+// length, which costs a wavefront unless its lanes hold rays of similar
+// length.  Consecutive merged points lie
+// within about one line of each other along an axis that ray_flags passed,
+// so the margin covers every segment of the tile.  Rays whose A list is not
+// ascending (angles outside [0, pi) where the quadrant rule and the slope
+// sign disagree), or whose primary axis fails, are walked whole.
+template <bool kExactSqrt, bool kTwoAxes, typename Visit>
 __device__ __forceinline__ int traverse_window(const GridInfo &g, const RayInfo &r, int quadrant,
                                                int tx0, int tx1, int ty0, int ty1,
                                                Visit &&visit) {
@@ -343,55 +368,62 @@ __device__ __forceinline__ int traverse_window(const GridInfo &g, const RayInfo 
     return walk<kExactSqrt>(w, g.mg, false, 0.f, 0.f, w.alen, w.blen, visit);
   }
   const int M = kWindowMargin;
-  if (__builtin_fabsf(r.islope) <= 1.0f) {
-    // window on A (rows): lines [ty0 - M, ty1 + M] as walk indices
-    int iS, iE;
-    if (w.fwd) {
-      iS = (ty0 - M) - w.alo;
-      iE = (ty1 + M) - w.alo + 1;
-    } else {
-      iS = (w.alo + w.alen - 1) - (ty1 + M);
-      iE = (w.alo + w.alen - 1) - (ty0 - M) + 1;
-    }
-    iS = max(0, min(iS, w.alen));
-    iE = max(0, min(iE, w.alen));
-    if (iS >= iE) return 0;
-    bool has_prev = false;
-    float px = 0.f, py = 0.f;
-    int j0 = 0;
-    if (iS > 0) {
-      const int n = w.a_line(iS - 1);
-      px = w.cross_a(n);
-      py = grid_line(g.N, n);
-      j0 = w.count_b_le(px);  // B elements merged before A[iS-1] (ties: B first)
-      has_prev = true;
-    }
-    w.seek(iS, j0);
-    const int i_stop = iE;
-    const int j_stop = (iE == w.alen) ? w.blen : 0;
-    return walk<kExactSqrt>(w, g.mg, has_prev, px, py, i_stop, j_stop, visit);
+  const bool on_a = __builtin_fabsf(r.islope) <= 1.0f;
+  const bool useA = on_a || (kTwoAxes && !w.one_axis);
+  const bool useB = !on_a || (kTwoAxes && !w.one_axis);
+  // A window (rows [ty0 - M, ty1 + M]) as walk indices [iS, iE)
+  int iS, iE;
+  if (w.fwd) {
+    iS = (ty0 - M) - w.alo;
+    iE = (ty1 + M) - w.alo + 1;
   } else {
-    // window on B (columns): lines [tx0 - M, tx1 + M]
-    int jS = (tx0 - M) - w.blo;
-    int jE = (tx1 + M) - w.blo + 1;
-    jS = max(0, min(jS, w.blen));
-    jE = max(0, min(jE, w.blen));
-    if (jS >= jE) return 0;
-    bool has_prev = false;
-    float px = 0.f, py = 0.f;
-    int i0 = 0;
-    if (jS > 0) {
-      const int n = w.blo + jS - 1;
-      px = grid_line(g.N, n);
-      py = w.cross_b(n);
-      i0 = w.count_a_lt(px);  // A elements merged before B[jS-1] (strict <)
-      has_prev = true;
-    }
-    w.seek(i0, jS);
-    const int i_stop = (jE == w.blen) ? w.alen : 0;
-    const int j_stop = jE;
-    return walk<kExactSqrt>(w, g.mg, has_prev, px, py, i_stop, j_stop, visit);
+    iS = (w.alo + w.alen - 1) - (ty1 + M);
+    iE = (w.alo + w.alen - 1) - (ty0 - M) + 1;
   }
+  iS = max(0, min(iS, w.alen));
+  iE = max(0, min(iE, w.alen));
+  // B window (columns [tx0 - M, tx1 + M]) as indices [jS, jE)
+  int jS = (tx0 - M) - w.blo;
+  int jE = (tx1 + M) - w.blo + 1;
+  jS = max(0, min(jS, w.blen));
+  jE = max(0, min(jE, w.blen));
+  if ((useA && iS >= iE) || (useB && jS >= jE)) return 0;
+  // Start just after the later of A[iS-1] and B[jS-1] in merge order (x
+  // ascending; on a tie B is merged first, so A's position is the later).
+  const bool hasA = useA && iS > 0, hasB = useB && jS > 0;
+  const int nA0 = w.a_line(hasA ? iS - 1 : 0), nB0 = w.blo + (hasB ? jS - 1 : 0);
+  const float sa = w.cross_a(nA0), sb = grid_line(g.N, nB0);
+  bool has_prev = false;
+  float px = 0.f, py = 0.f;
+  int i0 = 0, j0 = 0;
+  if (hasA && (!hasB || sa >= sb)) {
+    px = sa;
+    py = grid_line(g.N, nA0);
+    i0 = iS;
+    j0 = w.count_b_le(px);  // B elements merged before A[iS-1]
+    has_prev = true;
+  } else if (hasB) {
+    px = sb;
+    py = w.cross_b(nB0);
+    i0 = w.count_a_lt(px);  // A elements merged before B[jS-1] (strict <)
+    j0 = jS;
+    has_prev = true;
+  }
+  // Stop after the earlier of A[iE-1] and B[jE-1] (a window that reaches the
+  // end of its list does not bound the walk).
+  const bool endA = useA && iE < w.alen, endB = useB && jE < w.blen;
+  const float ea = w.cross_a(w.a_line(endA ? iE - 1 : 0));
+  const float eb = grid_line(g.N, w.blo + (endB ? jE - 1 : 0));
+  int i_stop = w.alen, j_stop = w.blen;
+  if (endA && (!endB || ea < eb)) {
+    i_stop = iE;
+    j_stop = 0;
+  } else if (endB) {
+    i_stop = 0;
+    j_stop = jE;
+  }
+  w.seek(i0, j0);
+  return walk<kExactSqrt>(w, g.mg, has_prev, px, py, i_stop, j_stop, visit);
 }
 
 }  // namespace trace_hip
