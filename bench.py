@@ -93,17 +93,27 @@ def cpu_baseline(N, P_full, threads, target_s=12.0):
     }
 
 
-def kernel_source_hash() -> str:
-    """sha256 over the HIP sources and headers of the library: counter
-    records apply only to the kernels they were measured on."""
+def _code_only(text: str) -> str:
+    """C/C++ source without comments and with whitespace runs collapsed, so
+    that editing a comment does not change the hash of the code."""
+    import re
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", " ", text)
+    return " ".join(text.split())
+
+
+def kernel_source_hash(d: str | None = None) -> str:
+    """sha256 over the code (comments stripped) of the HIP sources and
+    headers of the library: counter records apply only to the kernels they
+    were measured on."""
     import hashlib
-    d = os.path.join(ROOT, "trace_amd", "csrc")
+    d = d or os.path.join(ROOT, "trace_amd", "csrc")
     h = hashlib.sha256()
     for f in sorted(os.listdir(d)):
         if f.endswith((".hip", ".h")):
             h.update(f.encode())
-            with open(os.path.join(d, f), "rb") as fh:
-                h.update(fh.read())
+            with open(os.path.join(d, f), encoding="utf-8") as fh:
+                h.update(_code_only(fh.read()).encode())
     return h.hexdigest()[:16]
 
 

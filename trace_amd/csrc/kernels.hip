@@ -10,19 +10,19 @@
 //   den    [N][N]           A^T 1 (iteration invariant, one slice's geometry)
 //   rays   [P][C]           RayInfo: source point, slopes, trimmed ranges
 //
-// Forward projector: one lane per ray (p, c) and slice batch; the 64 lanes
-// of a wavefront are adjacent detector columns of one angle, so at every
-// step they gather from neighbouring pixels.  Each lane walks the whole ray
-// in the reference's order and accumulates y = sum x[idx]*len per slice, so
-// y is bit-identical to CalculateSimdata (sirt.cc:4-19).
+// Forward projector: 1, 2 or 4 lanes per ray (p, c) and slice batch, each
+// summing its share of the batch's slices; a wavefront holds a bundle of
+// adjacent angles x adjacent detector columns.  Each lane walks the whole
+// ray in the reference's order and accumulates y = sum x[idx]*len per
+// slice, so y is bit-identical to CalculateSimdata (sirt.cc:4-19).
 //
 // Backprojector: one workgroup per (image tile, slice batch) owns the tile's
 // accumulators in LDS for all angles.  Its lanes take the (angle, column)
 // pairs whose ray can touch the tile, walk only the part of the ray near the
-// tile (traverse_window) and add len*w into LDS with ds_add_f32; the tile
-// is then written to HBM with coalesced stores.  No global atomics, no
-// per-thread image replicas (the reference's DISP replicas + tree merge,
-// disp_engine_reduction.h:104-185).
+// tile (traverse_window) and add the fixed-point terms of len*w into LDS
+// with integer atomics; the tile is then written to HBM with coalesced
+// stores.  No global atomics, no per-thread image replicas (the reference's
+// DISP replicas + tree merge, disp_engine_reduction.h:104-185).
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
