@@ -31,12 +31,13 @@ def _setup(N, C, P=40, S=5, seed=0):
 
 @pytest.mark.parametrize("N,C", CASES)
 @pytest.mark.parametrize("center", ["half", "default"])
-def test_operators(gpu, oracle, N, C, center):
+@pytest.mark.parametrize("S", [5, 1])  # 1: the one-slice tiles' two-axis windows
+def test_operators(gpu, oracle, N, C, center, S):
     import torch
     from trace_amd import Projector
     ctr = C / 2.0 if center == "half" else 0.0
-    th, x, r = _setup(N, C)
-    P, S = th.size, x.shape[0]
+    th, x, r = _setup(N, C, S=S)
+    P = th.size
     with Projector(th, S, C, ctr, num_grids=N) as pj:
         y = pj.forward(torch.from_numpy(x).cuda()).cpu().numpy()
         z = pj.adjoint(torch.from_numpy(r).cuda()).cpu().numpy()
