@@ -54,9 +54,10 @@ struct GridInfo {
 
 // Per-ray constants: source point, slopes and the trimmed list ranges
 // (A: horizontal-line crossings n in [alo, alo+alen); B: vertical-line
-// crossings n in [blo, blo+blen)).  Computed once per geometry.  Two flag
-// bits ride on alo (see ray_flags): kRayWhole, the tile walks must take the
-// whole ray; kRayWraps, the ray has a segment with ix == N.
+// crossings n in [blo, blo+blen)).  Computed once per geometry.  Three
+// flag bits ride on alo (see ray_flags): kRayWhole, the tile walks must take
+// the whole ray; kRayWraps, the ray has a segment with ix == N; kRay1D, only
+// the primary axis may bound the ray's tile windows.
 struct RayInfo {
   float srcx, srcy, slope, islope;
   int alo, alen, blo, blen;
@@ -257,27 +258,8 @@ __device__ __forceinline__ int traverse_full(const GridInfo &g, const RayInfo &r
   return walk<kExactSqrt>(w, g.mg, false, 0.f, 0.f, w.alen, w.blen, visit);
 }
 
-// Flags of a ray, from one walk of its merged points.
-//
-// kRayWraps: some segment's midpoint rounds onto the right image edge, so
-// its pixel index ix == N is pixel 0 of the next row (the reference's
-// truncation, trace_utils.cc:177-185).  Generic rays never do: their
-// crossings carry the 1/16-pixel granularity of the far source point.
-//
-// kRayWhole: a window walk of the ray would not be exact.  The window of a tile is a
-// range of grid lines along one axis (rows for |islope| <= 1, where the
-// A points carry exact row coordinates, columns otherwise).  The points of
-// the other list carry a computed coordinate along that axis; it must lie
-// within half a line of the span of the two exact points it is merged
-// between (1.5 lines before the first or after the last exact point), so
-// that every segment with its midpoint in the tile lies between exact
-// points inside the window and its margin.  Generic rays are off by less
-// than 0.2 line.  A ray lying within float rounding of a grid line (an odd
-// C - N, or a centre on a half pixel, at an axis-aligned angle) can have a
-// crossing of the parallel lines computed tens of pixels away; such rays are
-// walked whole by the tile kernels.
-// An axis check per axis: the window of the second axis may be intersected
-// with the first one's (kRay1D is set when the second axis fails).
+// The window consistency check of ray_flags on one axis (on_a: rows, where
+// the A points are exact), fed the merged points in order.
 struct AxisCheck {
   bool on_a, have_exact = false, pending = false, safe = true;
   float last = 0.f, cmin = 0.f, cmax = 0.f;
@@ -305,6 +287,29 @@ struct AxisCheck {
   }
 };
 
+// Flags of a ray, from one walk of its merged points.
+//
+// kRayWraps: some segment's midpoint rounds onto the right image edge, so
+// its pixel index ix == N is pixel 0 of the next row (the reference's
+// truncation, trace_utils.cc:177-185).  Generic rays never do: their
+// crossings carry the 1/16-pixel granularity of the far source point.
+//
+// kRayWhole: a window walk of the ray would not be exact.  The window of a tile is a
+// range of grid lines along one axis (rows for |islope| <= 1, where the
+// A points carry exact row coordinates, columns otherwise).  The points of
+// the other list carry a computed coordinate along that axis; it must lie
+// within half a line of the span of the two exact points it is merged
+// between (1.5 lines before the first or after the last exact point), so
+// that every segment with its midpoint in the tile lies between exact
+// points inside the window and its margin.  Generic rays are off by less
+// than 0.2 line.  A ray lying within float rounding of a grid line (an odd
+// C - N, or a centre on a half pixel, at an axis-aligned angle) can have a
+// crossing of the parallel lines computed tens of pixels away; such rays are
+// walked whole by the tile kernels.
+//
+// kRay1D: the same check fails on the second axis (columns for |islope| <=
+// 1, rows otherwise), so the ray's windows must not be intersected with a
+// window on that axis.
 __device__ __forceinline__ int ray_flags(const GridInfo &g, const RayInfo &r, int quadrant) {
   Walker w;
   w.init(r, g.N, quadrant);
