@@ -102,18 +102,22 @@ def _code_only(text: str) -> str:
     return " ".join(text.split())
 
 
+# The device code: the kernels and what they include.  Host files (the
+# C-ABI, the batch set-up) choose launch shapes, which every counter record
+# names separately.
+KERNEL_SOURCES = ("internal.h", "kernels.hip", "traverse.h")
+
+
 def kernel_source_hash(d: str | None = None) -> str:
-    """sha256 over the code (comments stripped) of the HIP sources and
-    headers of the library: counter records apply only to the kernels they
-    were measured on."""
+    """sha256 over the code (comments stripped) of the kernel sources:
+    counter records apply only to the kernels they were measured on."""
     import hashlib
     d = d or os.path.join(ROOT, "trace_amd", "csrc")
     h = hashlib.sha256()
-    for f in sorted(os.listdir(d)):
-        if f.endswith((".hip", ".h")):
-            h.update(f.encode())
-            with open(os.path.join(d, f), encoding="utf-8") as fh:
-                h.update(_code_only(fh.read()).encode())
+    for f in KERNEL_SOURCES:
+        h.update(f.encode())
+        with open(os.path.join(d, f), encoding="utf-8") as fh:
+            h.update(_code_only(fh.read()).encode())
     return h.hexdigest()[:16]
 
 

@@ -65,9 +65,16 @@ int ensure_rays(trace_handle *h) {
 int ensure_count(trace_handle *h) {
   const TileShape &t = h->tile;
   if (h->count && h->count_shape.tw == t.tw && h->count_shape.th == t.th) return TRACE_OK;
+  // the one-slice tile kernel enumerates ceil(sqrt(tw^2 + th^2)) + 4 column
+  // ranks per angle (bp_tile_loop): every count must fit
+  int ranks = 0;
+  while (ranks * ranks < t.tw * t.tw + t.th * t.th) ++ranks;
+  ranks += 4;
   std::vector<int> cnt(h->geo.P);
-  for (int p = 0; p < h->geo.P; ++p)
+  for (int p = 0; p < h->geo.P; ++p) {
     cnt[p] = tile_count_bound(h->ang[p].sinq, h->ang[p].cosq, t.tw, t.th);
+    if (cnt[p] > ranks) return fail(TRACE_E_UNSUPPORTED, "tile column bound exceeds the rank count");
+  }
   if (!h->count) HIP_TRY(dev_alloc(&h->count, (size_t)h->geo.P));
   HIP_TRY(hipMemcpyAsync(h->count, cnt.data(), cnt.size() * sizeof(int), hipMemcpyHostToDevice,
                          h->stream));

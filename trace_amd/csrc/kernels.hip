@@ -1206,7 +1206,9 @@ hipError_t bp_mode(int mode, bool exact, const TileArgs &a, int nbatch, hipStrea
 
 int tile_count_bound(float sinq, float cosq, int TW, int TH) {
   // columns in an interval of length TW|sin| + TH|cos| + 2 (margins): at
-  // most floor(len) + 1; one more for rounding of the float projections
+  // most floor(len) + 1; one more for rounding of the float projections.
+  // At most ceil(sqrt(TW^2 + TH^2)) + 4, the column ranks bp_tile_loop
+  // enumerates for one-slice tiles (ensure_count checks it).
   const double len = TW * std::fabs((double)sinq) + TH * std::fabs((double)cosq) + 2.0;
   return (int)std::floor(len) + 2;
 }
