@@ -174,17 +174,42 @@ struct Walker {
     ax = cross_a(nA); ay = grid_line(N, nA);
     bx = grid_line(N, nB); by = cross_b(nB);
   }
-  // Number of A elements (walk order, assumed ascending) with x < v.
+  // Number of A elements (walk order, assumed ascending) with x < v: the
+  // first walk index whose crossing is >= v.  It is estimated from the line
+  // whose crossing lies at x = v and confirmed by comparing the crossings
+  // on either side (exact: the sequence is monotone); an estimate more than
+  // a few elements off (ill-conditioned near-vertical rays) falls back to a
+  // binary search.
   __device__ __forceinline__ int count_a_lt(float v) const {
+    const float t = (v - srcx) * slope + srcy + 0.5f * (float)N;
+    if (alen > 0 && __builtin_isfinite(t) && __builtin_fabsf(t) < 1.0e7f) {
+      const int n = (int)floorf(t);
+      int i = fwd ? n - alo : alo + alen - 1 - n;
+      i = max(0, min(i, alen));
+      for (int step = 0; step < 4; ++step) {
+        const bool below = (i == 0) || cross_a(a_line(i - 1)) < v;
+        const bool above = (i == alen) || !(cross_a(a_line(i)) < v);
+        if (below && above) return i;
+        i += below ? 1 : -1;
+      }
+    }
     int L = 0, H = alen;
     while (L < H) { int m = (L + H) >> 1; if (cross_a(a_line(m)) < v) L = m + 1; else H = m; }
     return L;
   }
-  // Number of B elements with x <= v.
+  // Number of B elements with x <= v.  The B elements are the exact grid
+  // lines blo .. blo + blen - 1, so the count is the last line at or left of
+  // v, found from floor(v + N/2) and corrected by exact comparisons (the
+  // float sum is off by less than one line).  NaN counts nothing.
   __device__ __forceinline__ int count_b_le(float v) const {
-    int L = 0, H = blen;
-    while (L < H) { int m = (L + H) >> 1; if (grid_line(N, blo + m) <= v) L = m + 1; else H = m; }
-    return L;
+    if (blen == 0 || !(v >= grid_line(N, blo))) return 0;
+    const int last = blo + blen - 1;
+    if (v >= grid_line(N, last)) return blen;
+    int n = (int)floorf(v + 0.5f * (float)N);
+    n = max(blo, min(n, last));
+    while (grid_line(N, n + 1) <= v) ++n;  // stops at last: v < gl(last)
+    while (grid_line(N, n) > v) --n;       // stops at blo: v >= gl(blo)
+    return n - blo + 1;
   }
   // Emits the next merged element: A wins only on a strict ax < bx
   // (trace_utils.cc:184).  Branch-free across the wavefront.
