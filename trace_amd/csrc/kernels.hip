@@ -242,9 +242,12 @@ struct TileArgs {
 };
 
 // Columns whose ray can touch tile [x0,x1]x[y0,y1] at angle p: the ray of
-// column c is the line q.(-sin, cos) = yi0 + c; its merged points lie within
-// 0.1 pixel of that line, so a 1-pixel margin on the tile's projection is
-// ample.
+// column c is the line q.(-sin, cos) = yi0 + c.  Its merged points lie
+// within 0.1 pixel of that line across it (a crossing's error lies along
+// the computed coordinate, which for an ill-conditioned crossing is nearly
+// the ray's own direction), so a margin of kColumnMargin pixels on the
+// tile's projection is ample.
+constexpr float kColumnMargin = 0.5f;
 __device__ __forceinline__ void tile_columns(const AngleInfo &a, float x0, float x1, float y0,
                                              float y1, float yi0, int &cmin, int &cmax) {
   const float s = a.sinq, c = a.cosq;
@@ -252,8 +255,8 @@ __device__ __forceinline__ void tile_columns(const AngleInfo &a, float x0, float
   const float t10 = -x1 * s + y0 * c, t11 = -x1 * s + y1 * c;
   const float tmin = fminf(fminf(t00, t01), fminf(t10, t11));
   const float tmax = fmaxf(fmaxf(t00, t01), fmaxf(t10, t11));
-  cmin = (int)ceilf(tmin - 1.0f - yi0);
-  cmax = (int)floorf(tmax + 1.0f - yi0);
+  cmin = (int)ceilf(tmin - kColumnMargin - yi0);
+  cmax = (int)floorf(tmax + kColumnMargin - yi0);
 }
 
 // LDS accumulator layout of the tile backprojector: one plane per slice,
@@ -1205,11 +1208,12 @@ hipError_t bp_mode(int mode, bool exact, const TileArgs &a, int nbatch, hipStrea
 // ------------------------------------------------------------ launchers --
 
 int tile_count_bound(float sinq, float cosq, int TW, int TH) {
-  // columns in an interval of length TW|sin| + TH|cos| + 2 (margins): at
-  // most floor(len) + 1; one more for rounding of the float projections.
+  // columns in an interval of length TW|sin| + TH|cos| + 2 kColumnMargin:
+  // at most floor(len) + 1; one more for rounding of the float projections.
   // At most ceil(sqrt(TW^2 + TH^2)) + 4, the column ranks bp_tile_loop
   // enumerates for one-slice tiles (ensure_count checks it).
-  const double len = TW * std::fabs((double)sinq) + TH * std::fabs((double)cosq) + 2.0;
+  const double len =
+      TW * std::fabs((double)sinq) + TH * std::fabs((double)cosq) + 2.0 * kColumnMargin;
   return (int)std::floor(len) + 2;
 }
 
