@@ -246,8 +246,12 @@ struct TileArgs {
 // within 0.1 pixel of that line across it (a crossing's error lies along
 // the computed coordinate, which for an ill-conditioned crossing is nearly
 // the ray's own direction), so a margin of kColumnMargin pixels on the
-// tile's projection is ample.
-constexpr float kColumnMargin = 0.5f;
+// tile's projection is ample.  The host's count bound allows a margin of
+// half a pixel; the float projections below carry errors below 0.01 pixel
+// for N <= 32768, far below the 0.05 pixel kept in hand on each side, so
+// their range never exceeds that bound.
+constexpr float kColumnMargin = 0.45f;
+constexpr double kCountMargin = 0.5;
 __device__ __forceinline__ void tile_columns(const AngleInfo &a, float x0, float x1, float y0,
                                              float y1, float yi0, int &cmin, int &cmax) {
   const float s = a.sinq, c = a.cosq;
@@ -1208,13 +1212,14 @@ hipError_t bp_mode(int mode, bool exact, const TileArgs &a, int nbatch, hipStrea
 // ------------------------------------------------------------ launchers --
 
 int tile_count_bound(float sinq, float cosq, int TW, int TH) {
-  // columns in an interval of length TW|sin| + TH|cos| + 2 kColumnMargin:
-  // at most floor(len) + 1; one more for rounding of the float projections.
+  // columns in an interval of length TW|sin| + TH|cos| + 2 kColumnMargin
+  // (plus rounding of the float projections, far below the 2 x 0.05 pixel
+  // that kCountMargin adds): at most floor(len) + 1.
   // At most ceil(sqrt(TW^2 + TH^2)) + 4, the column ranks bp_tile_loop
   // enumerates for one-slice tiles (ensure_count checks it).
   const double len =
-      TW * std::fabs((double)sinq) + TH * std::fabs((double)cosq) + 2.0 * kColumnMargin;
-  return (int)std::floor(len) + 2;
+      TW * std::fabs((double)sinq) + TH * std::fabs((double)cosq) + 2.0 * kCountMargin;
+  return (int)std::floor(len) + 1;
 }
 
 hipError_t launch_ray_info(const Geometry &g, RayInfo *rays, hipStream_t st) {
