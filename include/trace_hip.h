@@ -110,6 +110,18 @@ typedef struct trace_handle trace_handle_t;
                                  batches 8, 16 and 32 (2 lanes) and 16, 32
                                  (4 lanes) */
 
+#define TRACE_OPT_ORDERED_SUMS 6 /* order of the backprojected float sums:
+                                    0: every sum in 32-bit fixed point
+                                       (exact to ~1e-8 of a pixel's sum,
+                                       rounded once);
+                                    1 (default): den = A^T 1 and the PML/
+                                       APMLR num summed in the reference's
+                                       order (projection, column, segment:
+                                       sirt.cc:62, pml.cc:243-244), bit for
+                                       bit; SIRT/MLEM num in fixed point;
+                                    2: SIRT/MLEM num in the reference's order
+                                       too (sirt.cc:61, mlem.cc:56) */
+
 /* Kernel phases reported by trace_phase_times. */
 #define TRACE_PHASE_FORWARD 0 /* forward projection and per-ray weights */
 #define TRACE_PHASE_BACK 1    /* backprojection A^T w */
@@ -172,7 +184,8 @@ int trace_adjoint(trace_handle_t *h, const float *r, float *z);
 
 /* den = A^T 1 for one slice of this geometry into den [N][N] (device): the
  * column sums the reference accumulates as the second replica entry
- * (sirt.cc:62). */
+ * (sirt.cc:62), summed in its order (bit for bit) unless
+ * TRACE_OPT_ORDERED_SUMS is 0. */
 int trace_column_sums(trace_handle_t *h, float *den);
 
 /* Per-ray sums of one slice's rays into [P][C] device arrays (either may be

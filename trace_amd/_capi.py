@@ -28,6 +28,7 @@ TRACE_OPT_PHASE_TIMING = 2
 TRACE_OPT_BACKPROJECTOR = 3
 TRACE_OPT_FWD_BUNDLE = 4
 TRACE_OPT_FWD_LANES = 5
+TRACE_OPT_ORDERED_SUMS = 6
 PHASES = ("forward", "back", "update", "setup")
 
 

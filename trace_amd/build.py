@@ -17,7 +17,7 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtrace_hip.so")
-SOURCES = ["kernels.hip", "capi.hip", "batch.hip"]
+SOURCES = ["kernels.hip", "ordered.hip", "capi.hip", "batch.hip"]
 HEADERS = ["traverse.h", "internal.h", "handle.h"]
 
 

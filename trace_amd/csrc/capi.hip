@@ -102,7 +102,8 @@ int backproject(trace_handle *h, int mode, const float *w, float *z, int phase) 
 }
 
 int ensure_den(trace_handle *h) {
-  if (h->den_ready && h->den_exact == h->exact) return TRACE_OK;
+  const bool ord = h->ordered >= 1;
+  if (h->den_ready && h->den_exact == h->exact && h->den_ordered == ord) return TRACE_OK;
   if (int rc = backproject(h, kBpOutOnes, nullptr, h->den, TRACE_PHASE_SETUP)) return rc;
   // largest column sum: bounds every pixel's sum of segment lengths, which
   // sets the 32-bit fixed-point scale of the weights backprojector
@@ -117,8 +118,33 @@ int ensure_den(trace_handle *h) {
   std::memcpy(&m, &b, sizeof(m));
   // den itself carries a fixed-point error far below 2^-20 relative
   h->den_max = std::isfinite(m) && m > 0.f ? m * (1.0f + 0x1p-16f) : 1.0f;
+  // the den the updates divide by: the reference's float sum in its order
+  // (the fixed-point sum above is exact to 2^-20 but rounds differently)
+  if (ord)
+    TIMED(TRACE_PHASE_SETUP, launch_bp_ordered(h->geo, h->rays, kOrdDen, h->exact, nullptr,
+                                               nullptr, 1, 1, h->den, h->wraps, h->stream));
   h->den_ready = true;
   h->den_exact = h->exact;
+  h->den_ordered = ord;
+  return TRACE_OK;
+}
+
+// Slices per thread of the ordered backprojector: up to 8, a divisor of the
+// slice-batched copy's batch for kOrdNum.
+static int ordered_batch(int mode, int S, int xsb) {
+  int sb = 8;
+  while (sb > 1 && sb / 2 >= S) sb /= 2;
+  if (mode == kOrdNum)
+    while (sb > 1 && (xsb % sb) != 0) sb /= 2;
+  return sb;
+}
+
+int backproject_ordered(trace_handle *h, int mode, const float *w, float *z, int phase) {
+  if (int rc = ensure_rays(h)) return rc;
+  const int sb = ordered_batch(mode, h->geo.S, h->xb_sb);
+  if (mode == kOrdNum && !h->xb) return fail(TRACE_E_INVALID, "no slice-batched image copy");
+  TIMED(phase, launch_bp_ordered(h->geo, h->rays, mode, h->exact, w, h->xb, h->xb_sb, sb, z,
+                                 h->wraps, h->stream));
   return TRACE_OK;
 }
 
@@ -389,6 +415,10 @@ int trace_set_option(trace_handle_t *h, int option, int value) {
         return fail(TRACE_E_INVALID, "angles per bundle x lanes per ray exceeds 64");
       h->fwd_bundle = value;
       return TRACE_OK;
+    case TRACE_OPT_ORDERED_SUMS:
+      if (value < 0 || value > 2) return fail(TRACE_E_INVALID, "value must be 0, 1 or 2");
+      h->ordered = value;
+      return TRACE_OK;
     case TRACE_OPT_FWD_LANES:
       if (value != 1 && value != 2 && value != 4)
         return fail(TRACE_E_INVALID, "lanes per ray must be 1, 2 or 4");
@@ -410,6 +440,7 @@ int trace_get_option(trace_handle_t *h, int option, int *value) {
     case TRACE_OPT_BACKPROJECTOR: *value = h->backprojector; return TRACE_OK;
     case TRACE_OPT_FWD_BUNDLE: *value = h->fwd_bundle; return TRACE_OK;
     case TRACE_OPT_FWD_LANES: *value = h->fwd_lanes; return TRACE_OK;
+    case TRACE_OPT_ORDERED_SUMS: *value = h->ordered; return TRACE_OK;
     default: return fail(TRACE_E_UNSUPPORTED, "unknown option");
   }
 }
@@ -465,11 +496,18 @@ int trace_reconstruct(trace_handle_t *h, trace_alg_t alg, int iters,
   const int nslots = nbatch(S, sb) * sb;
   const int fmode = (alg == TRACE_SIRT) ? kFwdOutSIRT : kFwdOutEM;
 
+  // sums in the reference's order: PML/APMLR num unless ordered == 0, SIRT/
+  // MLEM num at ordered == 2 (den: ensure_den)
+  const bool ord_num = reg ? h->ordered >= 1 : h->ordered >= 2;
+  auto back = [&]() {
+    return ord_num ? backproject_ordered(h, reg ? kOrdNum : kOrdEM, h->w, h->z, TRACE_PHASE_BACK)
+                   : backproject(h, kBpOutSino, h->w, h->z, TRACE_PHASE_BACK);
+  };
   if (alg == TRACE_SIRT || alg == TRACE_MLEM) {
     TIMED(TRACE_PHASE_SETUP, launch_pack(recon, S, 0, 0.f, NN, nslots, sb, h->xb, st));
     for (int it = 0; it < iters; ++it) {
       if (int rc = forward(h, fmode, sino, h->w)) return rc;
-      if (int rc = backproject(h, kBpOutSino, h->w, h->z, TRACE_PHASE_BACK)) return rc;
+      if (int rc = back()) return rc;
       TIMED(TRACE_PHASE_UPDATE,
             launch_update_em(alg == TRACE_SIRT, recon, h->z, h->den, S, NN, sb, h->xb, st));
     }
@@ -480,9 +518,9 @@ int trace_reconstruct(trace_handle_t *h, trace_alg_t alg, int iters,
     TIMED(TRACE_PHASE_SETUP, launch_pack(cur, S, 0, 0.f, NN, nslots, sb, h->xb, st));
     for (int it = 0; it < iters; ++it) {
       if (int rc = forward(h, fmode, sino, h->w)) return rc;
-      if (int rc = backproject(h, kBpOutSino, h->w, h->z, TRACE_PHASE_BACK)) return rc;
+      if (int rc = back()) return rc;
       TIMED(TRACE_PHASE_UPDATE, launch_update_pml(cur, alt, h->z, h->den, S, g.N, prm->beta,
-                                                  sb, nslots, h->xb, st));
+                                                  ord_num, sb, nslots, h->xb, st));
       std::swap(cur, alt);
     }
     if (cur != recon)
@@ -503,9 +541,10 @@ int trace_reconstruct(trace_handle_t *h, trace_alg_t alg, int iters,
   TIMED(TRACE_PHASE_SETUP, launch_pack(pa, S + 2, off, 0.f, NN, nslots, sb, h->xb, st));
   for (int it = 0; it < iters; ++it) {
     if (int rc = forward(h, kFwdOutEM, sino, h->w)) return rc;
-    if (int rc = backproject(h, kBpOutSino, h->w, h->z, TRACE_PHASE_BACK)) return rc;
+    if (int rc = back()) return rc;
     TIMED(TRACE_PHASE_UPDATE, launch_update_apmlr(pa, pb, h->z, h->den, S, g.N, rp, w,
-                                                  prm->apmlr_compat, sb, nslots, h->xb, st));
+                                                  prm->apmlr_compat, ord_num, sb, nslots,
+                                                  h->xb, st));
     std::swap(pa, pb);
   }
   HIP_TRY(hipMemcpyAsync(recon, pa + NN, sizeof(float) * S * NN, hipMemcpyDeviceToDevice, st));
@@ -578,7 +617,11 @@ int trace_column_sums(trace_handle_t *h, float *den) {
   if (!h) return fail(TRACE_E_INVALID, "handle is null");
   if (!den) return fail(TRACE_E_INVALID, "null buffer");
   HIP_TRY(hipSetDevice(h->device));
-  return backproject(h, kBpOutOnes, nullptr, den, TRACE_PHASE_SETUP);
+  if (h->ordered == 0) return backproject(h, kBpOutOnes, nullptr, den, TRACE_PHASE_SETUP);
+  if (int rc = ensure_rays(h)) return rc;
+  TIMED(TRACE_PHASE_SETUP, launch_bp_ordered(h->geo, h->rays, kOrdDen, h->exact, nullptr,
+                                             nullptr, 1, 1, den, h->wraps, h->stream));
+  return TRACE_OK;
 }
 
 int trace_segment_counts(trace_handle_t *h, float *counts) {

@@ -109,6 +109,8 @@ struct trace_handle {
   float *den = nullptr;        // [N][N]
   bool den_ready = false;
   bool den_exact = true;
+  bool den_ordered = false;    // den summed in the reference's order
+  int ordered = 1;             // TRACE_OPT_ORDERED_SUMS
   float *xb = nullptr;         // [ceil(S/fwd_sb)][N][N][fwd_sb]
   int xb_sb = 0;
   float *w = nullptr;          // [P][C][S] per-ray weights
@@ -152,6 +154,9 @@ int ensure_den(trace_handle *h);
 int ensure_work(trace_handle *h, bool images, bool batched_copy = true);
 // A^T applied with the configured backprojector (see capi.hip).
 int backproject(trace_handle *h, int mode, const float *w, float *z, int phase);
+// Backprojected sums of w in the reference's order (kOrdNum: PML/APMLR num
+// with the x factor from the slice-batched copy xb; kOrdEM: SIRT/MLEM num).
+int backproject_ordered(trace_handle *h, int mode, const float *w, float *z, int phase);
 // TRACE_E_NONFINITE if v[0..n) holds NaN or Inf (synchronous).
 int check_finite(trace_handle *h, const float *v, int64_t n);
 

@@ -13,6 +13,11 @@ namespace trace_hip {
 enum Alg { kSIRT = 0, kMLEM = 1, kPML = 2, kAPMLR = 3 };
 enum FwdModeHost { kFwdOutY = 0, kFwdOutSIRT = 1, kFwdOutEM = 2 };
 enum BpModeHost { kBpOutSino = 0, kBpOutOnes = 1, kBpOutCount = 2 };
+// Backprojections summed in the reference's order (ordered.hip):
+//   kOrdDen  den_j = sum len                 (one slice, [N][N])
+//   kOrdNum  num_j = -sum (x_j * len) * w    (PML/APMLR, [S][N][N])
+//   kOrdEM   num_j = sum len * w             (SIRT/MLEM, [S][N][N])
+enum OrdMode { kOrdDen = 0, kOrdNum = 1, kOrdEM = 2 };
 
 // One slice stack's geometry on the device.
 struct Geometry {
@@ -107,6 +112,15 @@ struct BatchQueues {
   X(32, 32, 1) X(64, 64, 1) X(32, 32, 2) X(16, 16, 2) X(16, 16, 1)          \
   X(16, 16, 32) X(32, 32, 16) X(32, 16, 16) X(24, 24, 16)
 
+// Backprojection in the reference's summation order (ordered.hip): w
+// [P][C][S] weights, xb the slice-batched image the ray pass read (xsb
+// slices per slot batch; kOrdNum only), sb slices per thread (a divisor of
+// xsb for kOrdNum).
+hipError_t launch_bp_ordered(const Geometry &g, const RayInfo *rays, int mode, bool exact,
+                             const float *w, const float *xb, int xsb, int sb, float *out,
+                             bool wraps, hipStream_t st);
+bool ordered_sb_supported(int mode, int sb, int xsb);
+
 hipError_t launch_backproject(const Geometry &g, const RayInfo *rays, const int *count,
                               const TileShape &ts, int mode, bool exact, const float *w,
                               const unsigned *maxw, float maxlen, float denmax, float *z,
@@ -120,13 +134,15 @@ hipError_t launch_pack(const float *src, int nsrc, int off, float fill, int64_t 
 hipError_t launch_update_em(bool sirt, float *x, const float *z, const float *den, int S,
                             int64_t NN, int sb, float *xb, hipStream_t st);
 hipError_t launch_fill(float *p, int64_t n, float v, hipStream_t st);
+// num_direct: z holds num summed as the reference does (launch_bp_ordered,
+// kOrdNum); otherwise z = A^T w and num = -x * z.
 hipError_t launch_update_pml(const float *x, float *xn, const float *z, const float *den,
-                             int S, int N, float beta, int sb, int nslots, float *xb,
-                             hipStream_t st);
+                             int S, int N, float beta, bool num_direct, int sb, int nslots,
+                             float *xb, hipStream_t st);
 hipError_t launch_update_apmlr(const float *xpad, float *xnpad, const float *z,
                                const float *den, int S, int N, const RegParams &p,
-                               const StencilWeights &w, int compat, int sb, int nslots,
-                               float *xb, hipStream_t st);
+                               const StencilWeights &w, int compat, bool num_direct,
+                               int sb, int nslots, float *xb, hipStream_t st);
 hipError_t launch_batch_reset(const BatchQueues &q, hipStream_t st);
 hipError_t launch_batch_forward(const BatchSliceDev *d, const BatchQueues &q, int it,
                                 int bundle_angles, bool exact, int grid, hipStream_t st);

@@ -241,28 +241,6 @@ struct TileArgs {
   int wraps;          // some ray has a segment with ix == N (kRayWraps)
 };
 
-// Columns whose ray can touch tile [x0,x1]x[y0,y1] at angle p: the ray of
-// column c is the line q.(-sin, cos) = yi0 + c.  Its merged points lie
-// within 0.1 pixel of that line across it (a crossing's error lies along
-// the computed coordinate, which for an ill-conditioned crossing is nearly
-// the ray's own direction), so a margin of kColumnMargin pixels on the
-// tile's projection is ample.  The host's count bound allows a margin of
-// half a pixel; the float projections below carry errors below 0.01 pixel
-// for N <= 32768, far below the 0.05 pixel kept in hand on each side, so
-// their range never exceeds that bound.
-constexpr float kColumnMargin = 0.45f;
-constexpr double kCountMargin = 0.5;
-__device__ __forceinline__ void tile_columns(const AngleInfo &a, float x0, float x1, float y0,
-                                             float y1, float yi0, int &cmin, int &cmax) {
-  const float s = a.sinq, c = a.cosq;
-  const float t00 = -x0 * s + y0 * c, t01 = -x0 * s + y1 * c;
-  const float t10 = -x1 * s + y0 * c, t11 = -x1 * s + y1 * c;
-  const float tmin = fminf(fminf(t00, t01), fminf(t10, t11));
-  const float tmax = fmaxf(fmaxf(t00, t01), fmaxf(t10, t11));
-  cmin = (int)ceilf(tmin - kColumnMargin - yi0);
-  cmax = (int)floorf(tmax + kColumnMargin - yi0);
-}
-
 // LDS accumulator layout of the tile backprojector: one plane per slice,
 // rows padded to TW + 1 64-bit words.  Neighbouring rays of one angle sit
 // one pixel apart across their direction, so at a given step the lanes of a
@@ -755,7 +733,7 @@ template <int SB, bool kInner>
 __device__ __forceinline__ void pml_pixel(const float *__restrict__ x, float *__restrict__ xn,
                                           const float *__restrict__ z, int S, int N,
                                           int64_t NN, int64_t pix, int cls, float d,
-                                          float beta, int b, float (&v)[SB]) {
+                                          float beta, int num_direct, int b, float (&v)[SB]) {
   const float *w = c_pml_w[cls == 0 ? 0 : (cls <= 4 ? 1 : 2)];
   const int cnt = c_nbn[cls];
 #pragma unroll
@@ -783,7 +761,7 @@ __device__ __forceinline__ void pml_pixel(const float *__restrict__ x, float *__
       }
       const float G = gg + d;
       const float F = f;
-      const float nm = -(x0 * z[j]);
+      const float nm = num_direct ? z[j] : -(x0 * z[j]);
       const float xv = (-G + __builtin_sqrtf(G * G - 8.0f * nm * F)) / (4.0f * F);
       xn[j] = xv;
       v[q] = xv;
@@ -792,8 +770,10 @@ __device__ __forceinline__ void pml_pixel(const float *__restrict__ x, float *__
 }
 
 // Fused PML stencil (pml.cc:25-201, F and G from zero) and closed-form
-// update (pml.cc:3-23):  x' = (-G + sqrt(G^2 - 8 num F)) / (4F), with
-// num = -x * (A^T w) (the x factor of pml.cc:243 taken out of the sum).
+// update (pml.cc:3-23):  x' = (-G + sqrt(G^2 - 8 num F)) / (4F).  With
+// num_direct, z holds num as the reference sums it (ordered.hip); otherwise
+// z = A^T w and num = -x * z (the x factor of pml.cc:243 taken out of the
+// sum).
 // A thread owns pixel (m, n) of the SB slices of slot batch b and also
 // writes the slice-batched copy xb of the new image for the next ray pass
 // (slot t = slice t; padding slots hold 0).
@@ -801,7 +781,7 @@ template <int SB>
 __global__ void __launch_bounds__(kPixBlock)
 k_update_pml(const float *__restrict__ x, float *__restrict__ xn,
              const float *__restrict__ z, const float *__restrict__ den,
-             int S, int N, float beta, float *__restrict__ xb) {
+             int S, int N, float beta, int num_direct, float *__restrict__ xb) {
   const int m = blockIdx.x * kPixBlock + threadIdx.x;
   const int n = blockIdx.y;
   const int b = blockIdx.z;
@@ -812,9 +792,9 @@ k_update_pml(const float *__restrict__ x, float *__restrict__ xn,
   const float d = den[pix];
   float v[SB];
   if (cls == 0)
-    pml_pixel<SB, true>(x, xn, z, S, N, NN, pix, cls, d, beta, b, v);
+    pml_pixel<SB, true>(x, xn, z, S, N, NN, pix, cls, d, beta, num_direct, b, v);
   else
-    pml_pixel<SB, false>(x, xn, z, S, N, NN, pix, cls, d, beta, b, v);
+    pml_pixel<SB, false>(x, xn, z, S, N, NN, pix, cls, d, beta, num_direct, b, v);
   store_slices<SB>(xb + ((int64_t)b * NN + pix) * SB, v);
 }
 
@@ -822,7 +802,8 @@ struct ApmlrArgs {
   StencilWeights w;
   RegParams p;
   int compat;
-  int nslots;  // slots of the batched copy
+  int nslots;      // slots of the batched copy
+  int num_direct;  // z holds num (ordered.hip), not A^T w
 };
 
 // APMLR stencil and update of the slots of batch b at one pixel (see
@@ -883,7 +864,7 @@ __device__ __forceinline__ void apmlr_pixel(const float *__restrict__ xp,
       const float G = gg + d;
       const float F = f;
       const float xpass = a.compat ? xp[jp - NN] : x0;
-      const float nm = -(xpass * z[j]);
+      const float nm = a.num_direct ? z[j] : -(xpass * z[j]);
       val = (-G + __builtin_sqrtf(G * G - 8.0f * nm * F)) / (4.0f * F);
       xnp[jp] = val;
     } else if (pi == 0 || pi == S + 1) {
@@ -1317,13 +1298,13 @@ hipError_t launch_fill(float *p, int64_t n, float v, hipStream_t st) {
 }
 
 hipError_t launch_update_pml(const float *x, float *xn, const float *z, const float *den,
-                             int S, int N, float beta, int sb, int nslots, float *xb,
-                             hipStream_t st) {
+                             int S, int N, float beta, bool num_direct, int sb, int nslots,
+                             float *xb, hipStream_t st) {
   dim3 grid(blocks_for(N, kPixBlock), N, nslots / sb);
 #define TRACE_PML_CASE(SB_)                                                              \
   case SB_:                                                                              \
     hipLaunchKernelGGL(k_update_pml<SB_>, grid, dim3(kPixBlock), 0, st, x, xn, z, den, S, \
-                       N, beta, xb);                                                      \
+                       N, beta, num_direct ? 1 : 0, xb);                                  \
     break;
   switch (sb) {
     TRACE_PML_CASE(1)
@@ -1340,13 +1321,13 @@ hipError_t launch_update_pml(const float *x, float *xn, const float *z, const fl
 
 hipError_t launch_update_apmlr(const float *xpad, float *xnpad, const float *z,
                                const float *den, int S, int N, const RegParams &p,
-                               const StencilWeights &w, int compat, int sb, int nslots,
-                               float *xb, hipStream_t st) {
+                               const StencilWeights &w, int compat, bool num_direct,
+                               int sb, int nslots, float *xb, hipStream_t st) {
   // batches: every slot of the copy and every real slice's slot
   const int off = compat ? 0 : 1;
   const int need = (nslots > S + 1 - off) ? nslots : S + 1 - off;
   dim3 grid(blocks_for(N, kPixBlock), N, (need + sb - 1) / sb);
-  ApmlrArgs a{w, p, compat, nslots};
+  ApmlrArgs a{w, p, compat, nslots, num_direct ? 1 : 0};
 #define TRACE_APMLR_CASE(SB_)                                                         \
   case SB_:                                                                           \
     hipLaunchKernelGGL(k_update_apmlr<SB_>, grid, dim3(kPixBlock), 0, st, xpad, xnpad, z, \

@@ -456,4 +456,26 @@ __device__ __forceinline__ int traverse_window(const GridInfo &g, const RayInfo 
   return walk<kExactSqrt>(w, g.mg, has_prev, px, py, i_stop, j_stop, visit);
 }
 
+// Columns whose ray can touch tile [x0,x1]x[y0,y1] at angle p: the ray of
+// column c is the line q.(-sin, cos) = yi0 + c.  Its merged points lie
+// within 0.1 pixel of that line across it (a crossing's error lies along
+// the computed coordinate, which for an ill-conditioned crossing is nearly
+// the ray's own direction), so a margin of kColumnMargin pixels on the
+// tile's projection is ample.  The host's count bound allows a margin of
+// half a pixel; the float projections below carry errors below 0.01 pixel
+// for N <= 32768, far below the 0.05 pixel kept in hand on each side, so
+// their range never exceeds that bound.
+constexpr float kColumnMargin = 0.45f;
+constexpr double kCountMargin = 0.5;
+__device__ __forceinline__ void tile_columns(const AngleInfo &a, float x0, float x1, float y0,
+                                             float y1, float yi0, int &cmin, int &cmax) {
+  const float s = a.sinq, c = a.cosq;
+  const float t00 = -x0 * s + y0 * c, t01 = -x0 * s + y1 * c;
+  const float t10 = -x1 * s + y0 * c, t11 = -x1 * s + y1 * c;
+  const float tmin = fminf(fminf(t00, t01), fminf(t10, t11));
+  const float tmax = fmaxf(fmaxf(t00, t01), fmaxf(t10, t11));
+  cmin = (int)ceilf(tmin - kColumnMargin - yi0);
+  cmax = (int)floorf(tmax + kColumnMargin - yi0);
+}
+
 }  // namespace trace_hip

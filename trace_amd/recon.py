@@ -143,6 +143,12 @@ class Projector:
         check(lib().trace_set_option(self._h, _capi.TRACE_OPT_FWD_LANES, int(lanes)))
         self._fwd_lanes = int(lanes)
 
+    def set_ordered_sums(self, level: int):
+        """Order of the backprojected float sums (TRACE_OPT_ORDERED_SUMS):
+        0 all in fixed point; 1 (default) den and the PML/APMLR num in the
+        reference's order; 2 also the SIRT/MLEM num."""
+        check(lib().trace_set_option(self._h, _capi.TRACE_OPT_ORDERED_SUMS, int(level)))
+
     def get_option(self, option: int) -> int:
         v = ct.c_int()
         check(lib().trace_get_option(self._h, int(option), ct.byref(v)))
