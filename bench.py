@@ -11,21 +11,29 @@ A step is one full iteration over the whole stack (zero accumulators, the
 ray pass A x -> weights -> A^T w, image update).  `value` is slice-iterations
 per second (= iterations per second per 2048^2 slice, summed over slices).
 
-`roofline` reports the dominant kernel of the step.  Algorithmic bytes per
-slice-iteration follow the stored-operator model B = 16 nnz + 12 R + 24 N^2
-(nnz = exact segment count of one slice, counted by the traversal; R = P x C
-rays), split over the three kernels of an iteration:
-  forward projector  8 nnz + 8 R + 4 N^2  (A once, read p, write w, read x)
-  backprojector      8 nnz + 4 R + 4 N^2  (A once, read w, write z)
-  image update       16 N^2               (read x, z, den; write x)
-`achieved` = the dominant kernel's bytes per launch (x slices) / its
-average launch time, measured with HIP events on the launch stream; peak
-8 TB/s.  The kernels regenerate A on the fly, so `frac` can exceed 1.
+`roofline` reports the dominant kernel of the step against the HBM roofline
+(8 TB/s).  The ray kernels regenerate the system matrix on the fly, so no
+byte model of a stored operator bounds them; what they move is measured:
+  achieved  = HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+              calibrated on a copy kernel; profiles/traffic.json) / the
+              kernel's average launch time, timed live with HIP events on
+              the launch stream;
+  frac      = achieved / 8 TB/s (<= 1 by construction);
+  traffic   = those HBM bytes per launch;
+  compulsory_bytes_per_launch = what the kernel cannot avoid moving: its
+              image, sinogram and weight arrays once, and the ray table once
+              per slice batch; traffic_over_compulsory is the re-read factor;
+  limiter   = the binding resource found from counters (profiles/
+              counters.json) with its utilisation;
+  stored_operator_equiv_GBs = the byte model of streaming a stored A once
+              per product (8 B per nonzero: utils/inters.py:11-15 counts the
+              nonzeros) over the same time, reported for comparison only.
 
 `cpu_baseline` times the reference's own threaded CPU engine (compiled from
 its sources into oracle/_ref) on this host: one MLEM iteration of one
 2048^2 slice over an evenly spaced subset of the angles, scaled linearly to
-the full angle count.
+the full angle count, with as many threads as the host gives this process
+(the reference's -t 0 takes hardware_concurrency, disp_engine_base.h:89-114).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -47,6 +55,41 @@ HBM_PEAK_GBS = 8000.0
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def host_cpus():
+    """(threads to use, CPUs in the affinity set, cgroup CPU quota or None,
+    online CPUs): the CPUs this process may run on, capped by a cgroup CPU
+    quota where one is set."""
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    for path, parse in (("/sys/fs/cgroup/cpu.max", lambda t: t.split()),
+                        ("/sys/fs/cgroup/cpu/cpu.cfs_quota_us", None)):
+        try:
+            if parse:
+                q, per = parse(open(path).read())
+                if q != "max":
+                    quota = int(q) / int(per)
+            else:
+                q = int(open(path).read())
+                per = int(open("/sys/fs/cgroup/cpu/cpu.cfs_period_us").read())
+                if q > 0:
+                    quota = q / per
+            break
+        except Exception:
+            continue
+    threads = aff if quota is None else max(1, min(aff, int(quota)))
+    return threads, aff, quota, os.cpu_count()
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(N, P_full, threads, target_s=12.0):
@@ -80,6 +123,7 @@ def cpu_baseline(N, P_full, threads, target_s=12.0):
     tb, _ = run(pa, 2)
     per_iter_sample = tb - ta  # one iteration, setup cancelled
     per_iter_full = per_iter_sample * P_full / pa
+    _, aff, quota, online = host_cpus()
     return {
         "value": 1.0 / per_iter_full,
         "unit": "slice-iter/s",
@@ -90,7 +134,29 @@ def cpu_baseline(N, P_full, threads, target_s=12.0):
                    f"iteration time = t(2 iters) - t(1 iter) = {per_iter_sample:.2f} s, "
                    f"scaled x{P_full / pa:.2f} to all angles"),
         "seconds_per_slice_iter": per_iter_full,
+        "host": {"cpu_model": cpu_model(), "online_cpus": online, "affinity_cpus": aff,
+                 "cgroup_cpu_quota": quota, "threads": threads,
+                 "compiler": R.compiler(),
+                 "flags": "-O3 -std=c++11 -ffp-contract=off (no fast-math, no -march)"},
     }
+
+
+def counter_record(kernel, workload, shape=None):
+    """Utilisation counters of a kernel (profiles/counters.json) measured on
+    the current kernel sources, or None."""
+    p = os.path.join(ROOT, "profiles", "counters.json")
+    try:
+        with open(p) as f:
+            recs = json.load(f)["records"]
+    except Exception:
+        return None
+    src = kernel_source_hash()
+    for r in recs:
+        if (r.get("kernel") == kernel and r.get("workload") == workload
+                and r.get("source_sha") == src
+                and (shape is None or r.get("shape") == list(shape))):
+            return r
+    return None
 
 
 def _code_only(text: str) -> str:
@@ -168,6 +234,11 @@ def main():
                          "uneven slices in one graph launch vs per-slice launches")
     ap.add_argument("--batch-slices", type=int, default=200,
                     help="batched: number of uneven slices")
+    ap.add_argument("--batched-iters", type=int, default=5,
+                    help="stack: iterations of the batched-vs-plain comparison "
+                         "appended to the line (200 uneven slices)")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="stack: skip the batched-vs-plain comparison")
     ap.add_argument("--iters", type=int, default=0,
                     help="pml/apmlr: iterations per step (default 30); autograd: "
                          "Adam steps (default 100)")
@@ -277,38 +348,56 @@ def run_stack(args):
     nonneg = bool((recon >= 0).all().item()) if args.alg == "mlem" else None
 
     R = P * N
-    model = {
-        "forward": ("k_forward", S * (8.0 * nnz + 8.0 * R + 4.0 * N * N)),
-        "back": ("k_bp_tile", S * (8.0 * nnz + 4.0 * R + 4.0 * N * N)),
-        "update": ("k_update_em", S * 16.0 * N * N),
-    }
     fb, tw, th, tb = pj.launch_shape()
     lanes, bundle = pj.fwd_lanes(), pj.fwd_bundle()
+    nb_f, nb_b = -(-S // fb), -(-S // tb)
+    # compulsory bytes per launch: each array once, the 32-byte ray table once
+    # per slice batch, den once per update batch
+    model = {
+        "forward": ("k_forward", S * (4.0 * N * N + 8.0 * R) + nb_f * 32.0 * R,
+                    S * (8.0 * nnz + 8.0 * R + 4.0 * N * N)),
+        "back": ("k_bp_tile", S * (4.0 * R + 4.0 * N * N) + nb_b * 32.0 * R,
+                 S * (8.0 * nnz + 4.0 * R + 4.0 * N * N)),
+        "update": ("k_update_em", S * 16.0 * N * N + nb_f * 4.0 * N * N, S * 16.0 * N * N),
+    }
     shapes = {"forward": [fb, lanes, bundle], "back": [tw, th, tb], "update": [fb]}
     per_kernel = {}
-    for ph, (kname, nbytes) in model.items():
+    for ph, (kname, comp, stored) in model.items():
         ms, n = phases[ph]
         t = ms / max(n, 1) / 1e3
-        per_kernel[ph] = {"kernel": kname, "avg_launch_ms": t * 1e3, "launches": n,
-                          "algorithmic_bytes_per_launch": nbytes,
-                          "achieved_GBs": nbytes / t / 1e9 if t > 0 else None,
-                          "traffic_bytes_per_launch": traffic_record(kname, workload,
-                                                                     shapes[ph])}
+        traffic = traffic_record(kname, workload, shapes[ph])
+        per_kernel[ph] = {
+            "kernel": kname, "avg_launch_ms": t * 1e3, "launches": n,
+            "traffic_bytes_per_launch": traffic,
+            "hbm_GBs": traffic / t / 1e9 if (traffic and t > 0) else None,
+            "compulsory_bytes_per_launch": comp,
+            "traffic_over_compulsory": traffic / comp if traffic else None,
+            "stored_operator_equiv_GBs": stored / t / 1e9 if t > 0 else None,
+        }
     dom = max(("forward", "back"), key=lambda k: per_kernel[k]["avg_launch_ms"])
-    kernel = per_kernel[dom]["kernel"]
-    t_dom = per_kernel[dom]["avg_launch_ms"] / 1e3
-    bytes_dom = per_kernel[dom]["algorithmic_bytes_per_launch"]
-    achieved = bytes_dom / t_dom / 1e9
-    traffic = per_kernel[dom]["traffic_bytes_per_launch"]
-    b_alg_step = S * (16.0 * nnz + 12.0 * R + 24.0 * N * N)
+    kd = per_kernel[dom]
+    kernel = kd["kernel"]
+    t_dom = kd["avg_launch_ms"] / 1e3
+    traffic = kd["traffic_bytes_per_launch"]
+    achieved = kd["hbm_GBs"]
+    crec = counter_record(kernel, workload, shapes[dom])
+    limiter = None
+    if crec:
+        limiter = {"resource": crec.get("limiter"), "utilisation": crec.get("derived"),
+                   "source": crec.get("source")}
+    b_stored_step = S * (16.0 * nnz + 12.0 * R + 24.0 * N * N)
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or host_cpus()[0]
         log(f"cpu baseline with {threads} threads ...")
         cpu = cpu_baseline(N, P, threads)
         if cpu:
             log(f"cpu baseline: {cpu['sample']}")
+
+    batched = None
+    if not args.no_batched:
+        batched = batched_compare(args.batch_slices, args.batched_iters)
 
     value = S * args.steps / elapsed
     out = {
@@ -341,12 +430,16 @@ def run_stack(args):
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
             "traffic": traffic,
-            "algorithmic_bytes_per_launch": bytes_dom,
+            "achieved_from": "counter HBM bytes per launch / HIP-event launch time",
             "avg_launch_ms": t_dom * 1e3,
+            "compulsory_bytes_per_launch": kd["compulsory_bytes_per_launch"],
+            "traffic_over_compulsory": kd["traffic_over_compulsory"],
+            "limiter": limiter,
+            "stored_operator_equiv_GBs": kd["stored_operator_equiv_GBs"],
+            "stored_operator_step_equiv_GBs": b_stored_step / (elapsed / args.steps) / 1e9,
             "nnz_per_slice": nnz,
-            "step_model_frac": b_alg_step / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS,
         },
         "kernels": per_kernel,
         "launch_shape": {"fwd_batch_tile_w_h_batch": list(pj.launch_shape()),
@@ -356,6 +449,7 @@ def run_stack(args):
         "phases_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in phases.items()},
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "batched": batched,
         "checks": {"finite": finite, "nonnegative": nonneg},
     }
     print(json.dumps(out), flush=True)
@@ -575,28 +669,26 @@ def run_autograd(args):
     }
     print(json.dumps(out), flush=True)
 
-def run_batched(args):
+def batched_compare(n, iters):
     """Uneven slices (widths {256..2048}, 180-1800 angles, SIRT or MLEM,
-    `--iters` iterations, default 25) reconstructed (a) slice by slice with
-    the plain per-slice entry point, each slice with the shape the
-    autotuner picks for its width, and (b) all together by one graph launch
-    of the batched persistent kernels with the shape the batch autotuner
-    picks.  Setup (tables, graph instantiation) is outside both timings;
-    every slice's two results are compared bit for bit."""
+    `iters` iterations) reconstructed (a) slice by slice with the plain
+    per-slice entry point, each slice with the shape the autotuner picks for
+    its width, and (b) all together by one graph launch of the batched
+    persistent kernels with the shape the batch autotuner picks.  Setup
+    (tables, graph instantiation) is outside both timings; every slice's two
+    results are compared bit for bit.  Returns a dict."""
     import torch
 
     from trace_amd import Projector
     from trace_amd.autotune import autotune
     from trace_amd.batch import Batch, autotune_batch, uneven_slices, uneven_specs
 
-    n = args.batch_slices
-    iters = args.iters or 25
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     specs = uneven_specs(n, seed=3)
     workload = (f"{n} uneven slices (width 256-2048, 180-1800 angles, SIRT/MLEM, seed 3), "
                 f"{iters} iterations each")
-    log(f"workload: {workload}")
+    log(f"batched: {workload}")
     t0 = time.perf_counter()
     slices = uneven_slices(specs, seed=3, iters=iters, device=0)
     torch.cuda.synchronize()
@@ -665,36 +757,53 @@ def run_batched(args):
             d = ((xb.double() - x2.double()).norm() / x2.double().norm().clamp_min(1e-30)).item()
             worst = max(worst, d)
     b.close()
-    out = {
-        "metric": "batched uneven slices/s (one graph launch) vs plain per-slice launches",
-        "value": n / t_batch,
-        "unit": "slices/s",
-        "n_gpus": 1,
-        "steps": 1,
-        "warmup": 0,
-        "ms_per_step": t_batch * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic: seeded ellipse phantoms (seed 3, 20 ellipses), Poisson 1e4",
-        "config": {"workload": workload, "slices": n, "iters": iters,
-                   "widths": {str(c): sum(1 for sp in specs if sp[0] == c)
-                              for c in sorted({sp[0] for sp in specs})},
-                   "sirt": sum(1 for sp in specs if sp[2] == "sirt")},
-        "plain": {"seconds": t_plain, "slices_per_s": n / t_plain,
-                  "shapes": {str(k): {"shape": v["shape"], "bundle": v["bundle"],
-                                      "lanes": v.get("lanes", 1)}
-                             for k, v in shapes.items()}},
-        "batched": {"seconds": t_batch, "bundle": tune["bundle"], "tile": list(tune["tile"]),
-                    "autotune_ms": tune["ms"]},
-        "speedup_vs_plain": t_plain / t_batch,
+    return {
+        "workload": workload, "slices": n, "iters": iters,
+        "widths": {str(c): sum(1 for sp in specs if sp[0] == c)
+                   for c in sorted({sp[0] for sp in specs})},
+        "sirt": sum(1 for sp in specs if sp[2] == "sirt"),
+        "plain_s": t_plain, "batched_s": t_batch, "ratio": t_plain / t_batch,
+        "plain_shapes": {str(k): {"shape": v["shape"], "bundle": v["bundle"],
+                                  "lanes": v.get("lanes", 1)} for k, v in shapes.items()},
+        "batched_shape": {"bundle": tune["bundle"], "tile": list(tune["tile"])},
+        "batched_autotune_ms": tune["ms"],
         "slice_iter_per_s": n * iters / t_batch,
         "segments_per_s": nnz_total * iters * 2 / t_batch,
         "parity": {"bit_identical_slices": identical, "slices": n,
                    "worst_rel_l2_of_others": worst},
     }
+
+
+def run_batched(args):
+    """Config 3 on its own line: 200 uneven slices, 25 iterations each
+    (batched_compare)."""
+    r = batched_compare(args.batch_slices, args.iters or 25)
+    out = {
+        "metric": "batched uneven slices/s (one graph launch) vs plain per-slice launches",
+        "value": r["slices"] / r["batched_s"],
+        "unit": "slices/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": r["batched_s"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded ellipse phantoms (seed 3, 20 ellipses), Poisson 1e4",
+        "config": {"workload": r["workload"], "slices": r["slices"], "iters": r["iters"],
+                   "widths": r["widths"], "sirt": r["sirt"]},
+        "plain": {"seconds": r["plain_s"], "slices_per_s": r["slices"] / r["plain_s"],
+                  "shapes": r["plain_shapes"]},
+        "batched": {"seconds": r["batched_s"], **r["batched_shape"],
+                    "autotune_ms": r["batched_autotune_ms"]},
+        "speedup_vs_plain": r["ratio"],
+        "slice_iter_per_s": r["slice_iter_per_s"],
+        "segments_per_s": r["segments_per_s"],
+        "parity": r["parity"],
+    }
     print(json.dumps(out), flush=True)
+
 
 if __name__ == "__main__":
     main()

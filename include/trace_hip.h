@@ -121,6 +121,13 @@ typedef struct trace_handle trace_handle_t;
                                        bit; SIRT/MLEM num in fixed point;
                                     2: SIRT/MLEM num in the reference's order
                                        too (sirt.cc:61, mlem.cc:56) */
+#define TRACE_OPT_ORDERED_TILE 7 /* form of the ordered sums (same bits either
+                                    way): 1 (default) LDS-slotted 8x8 tiles
+                                    where the geometry allows (no segment
+                                    wrapping to the next row, at most two
+                                    segments per ray and pixel; checked once
+                                    per geometry), else per pixel; 0 always
+                                    per pixel */
 
 /* Kernel phases reported by trace_phase_times. */
 #define TRACE_PHASE_FORWARD 0 /* forward projection and per-ray weights */
@@ -276,6 +283,17 @@ int trace_batch_reconstruct(const trace_batch_slice_t *slices, int n, int device
                             void *hip_stream);
 
 void trace_batch_destroy(trace_batch_t *b);
+
+/* ---- Testing hooks (no reference counterpart) ------------------------
+ * The reference ignores allocation failures (new throws, hipMalloc does not
+ * exist there); these hooks let a test force the library's out-of-memory
+ * paths from the host.  trace_debug_fail_alloc(k): the library's device
+ * allocations succeed k more times, then one fails with hipErrorOutOfMemory
+ * (reported as TRACE_E_NOMEM) and the hook switches itself off; k < 0
+ * switches it off.  Process-wide.  trace_debug_live_allocations(): device
+ * allocations of the library currently live, process-wide. */
+void trace_debug_fail_alloc(int after);
+int64_t trace_debug_live_allocations(void);
 
 /* Thread-local text of the last error ("" if none). */
 const char *trace_last_error(void);

@@ -243,6 +243,8 @@ class RefLib:
         L.ref_quadrant.restype = ct.c_int
         L.ref_mov.argtypes = [ct.c_int, ct.c_float]
         L.ref_mov.restype = ct.c_float
+        L.ref_compiler.argtypes = []
+        L.ref_compiler.restype = ct.c_char_p
         common = [ct.c_int, ct.c_int, ct.c_int, _fp, ct.c_float, ct.c_int,
                   ct.c_int, _fp, _fp]
         L.ref_sirt.argtypes = common + [_fp, _fp]
@@ -287,6 +289,9 @@ class RefLib:
 
     def quadrant(self, q):
         return self.L.ref_quadrant(float(q))
+
+    def compiler(self) -> str:
+        return self.L.ref_compiler().decode()
 
     def mov(self, C, center):
         return self.L.ref_mov(C, center)

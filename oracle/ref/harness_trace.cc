@@ -124,4 +124,7 @@ float ref_mov(int C, float center) {
   return md.mov();
 }
 
+// Compiler that built this library (reported beside CPU timings).
+const char *ref_compiler(void) { return "g++ " __VERSION__; }
+
 }  // extern "C"

@@ -31,13 +31,15 @@ def _den_oracle(oracle, th, N, C, center):
     return st["den"][0]
 
 
+@pytest.mark.parametrize("tile", [True, False])
 @pytest.mark.parametrize("N,C,P,center", GEOMS)
-def test_column_sums_bit_exact(gpu, oracle, N, C, P, center):
+def test_column_sums_bit_exact(gpu, oracle, N, C, P, center, tile):
     from trace_amd import Projector
     from trace_amd.phantom import angles_rad
     th = angles_rad(P)
-    print(f"ordered den N={N} C={C} P={P} centre={center}", flush=True)
+    print(f"ordered den N={N} C={C} P={P} centre={center} tile={tile}", flush=True)
     with Projector(th, 1, C, center, num_grids=N) as pj:
+        pj.set_ordered_tile(tile)
         d = pj.column_sums().cpu().numpy()
     do = _den_oracle(oracle, th, N, C, center)
     bad = int(np.count_nonzero(d.view(np.uint32) != do.view(np.uint32)))
@@ -45,7 +47,7 @@ def test_column_sums_bit_exact(gpu, oracle, N, C, P, center):
     assert bad == 0
 
 
-def _recon(alg, th, sino, init, center, iters, prm, level, N):
+def _recon(alg, th, sino, init, center, iters, prm, level, N, tile=True):
     import torch
     from trace_amd import Projector, RegParams
     P, S, C = sino.shape
@@ -53,6 +55,7 @@ def _recon(alg, th, sino, init, center, iters, prm, level, N):
                    bool(prm.apmlr_compat))
     with Projector(th, S, C, center, num_grids=N) as pj:
         pj.set_ordered_sums(level)
+        pj.set_ordered_tile(tile)
         ds = torch.from_numpy(np.ascontiguousarray(sino, np.float32)).cuda()
         dx = torch.from_numpy(np.ascontiguousarray(init, np.float32)).cuda()
         pj.reconstruct(alg, iters, ds, dx, rp)
@@ -72,11 +75,14 @@ CASES = [
     ("mlem", 64, 64, 48, 3, 6, 32.0, 1, 2),
     ("mlem", 64, 48, 40, 2, 4, 24.0, 1, 2),
     ("pml", 256, 256, 300, 1, 5, 128.0, 1, 1),
+    ("pml", 96, 96, 64, 40, 3, 48.0, 1, 1),   # 32-slice batches of the tile form
+    ("mlem", 72, 72, 50, 33, 3, 36.0, 1, 2),  # 32 + 1
 ]
 
 
+@pytest.mark.parametrize("tile", [True, False])
 @pytest.mark.parametrize("alg,N,C,P,S,iters,center,compat,level", CASES)
-def test_recon_bit_exact(gpu, oracle, alg, N, C, P, S, iters, center, compat, level):
+def test_recon_bit_exact(gpu, oracle, alg, N, C, P, S, iters, center, compat, level, tile):
     from trace_amd.phantom import angles_rad, noisy_sinogram
     th = angles_rad(P)
     sino = noisy_sinogram(th, C, S, seed=N + S + P, I0=1e4, K=8).numpy()
@@ -84,7 +90,7 @@ def test_recon_bit_exact(gpu, oracle, alg, N, C, P, S, iters, center, compat, le
     prm = Params(beta=0.1, beta1=1.0, delta=0.01, delta1=1.0, regw=1.0, apmlr_compat=compat)
     print(f"ordered {alg} N={N} C={C} P={P} S={S} it={iters} c={center} compat={compat}",
           flush=True)
-    x = _recon(alg, th, sino, init, center, iters, prm, level, N)
+    x = _recon(alg, th, sino, init, center, iters, prm, level, N, tile)
     xo = oracle.reconstruct(alg, th, sino, init, center, iters, prm)
     bad = int(np.count_nonzero(x.view(np.uint32) != xo.view(np.uint32)))
     print(f"{alg}: {bad} of {x.size} pixels differ, rel L2 {rel_l2(x, xo):.2e}")

@@ -29,6 +29,7 @@ TRACE_OPT_BACKPROJECTOR = 3
 TRACE_OPT_FWD_BUNDLE = 4
 TRACE_OPT_FWD_LANES = 5
 TRACE_OPT_ORDERED_SUMS = 6
+TRACE_OPT_ORDERED_TILE = 7
 PHASES = ("forward", "back", "update", "setup")
 
 
@@ -135,6 +136,8 @@ _SIGS = {
     "trace_batch_prepare": (ct.c_int, [_vp, ct.c_int]),
     "trace_batch_reconstruct": (ct.c_int, [ct.POINTER(TraceBatchSlice), ct.c_int, ct.c_int, _vp]),
     "trace_batch_destroy": (None, [_vp]),
+    "trace_debug_fail_alloc": (None, [ct.c_int]),
+    "trace_debug_live_allocations": (ct.c_int64, []),
     "trace_last_error": (ct.c_char_p, []),
     "trace_version": (ct.c_char_p, []),
 }

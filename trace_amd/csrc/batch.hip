@@ -230,7 +230,7 @@ void trace_batch_destroy(trace_batch_t *b) {
   for (trace_handle_t *h : b->h) trace_destroy(h);
   for (void *p : {(void *)b->d_desc, (void *)b->d_tables, (void *)b->counters, (void *)b->maxw,
                   (void *)b->flag})
-    (void)hipFree(p);
+    dev_free(p);
   if (b->ev_in) (void)hipEventDestroy(b->ev_in);
   if (b->ev_out) (void)hipEventDestroy(b->ev_out);
   if (b->cap) (void)hipStreamDestroy(b->cap);

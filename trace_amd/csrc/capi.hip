@@ -10,6 +10,7 @@
 //   image update        x' from x, z, den [, stencil]    (k_update_*)
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -28,7 +29,27 @@ namespace trace_hip {
 
 namespace {
 thread_local std::string g_err;
+std::atomic<long long> g_live_allocs{0};
+std::atomic<int> g_fail_after{-1};  // successful allocations before one fails
 }  // namespace
+
+hipError_t device_alloc(void **p, size_t bytes) {
+  *p = nullptr;
+  int k = g_fail_after.load();
+  while (k >= 0) {
+    if (g_fail_after.compare_exchange_weak(k, k - 1)) {
+      if (k == 0) return hipErrorOutOfMemory;  // the injected failure (then off)
+      break;
+    }
+  }
+  const hipError_t e = hipMalloc(p, bytes);
+  if (e == hipSuccess && *p) g_live_allocs.fetch_add(1);
+  return e;
+}
+
+void dev_free(void *p) {
+  if (p && hipFree(p) == hipSuccess) g_live_allocs.fetch_sub(1);
+}
 
 int fail(int code, const std::string &msg) {
   g_err = msg;
@@ -121,30 +142,55 @@ int ensure_den(trace_handle *h) {
   // the den the updates divide by: the reference's float sum in its order
   // (the fixed-point sum above is exact to 2^-20 but rounds differently)
   if (ord)
-    TIMED(TRACE_PHASE_SETUP, launch_bp_ordered(h->geo, h->rays, kOrdDen, h->exact, nullptr,
-                                               nullptr, 1, 1, h->den, h->wraps, h->stream));
+    if (int rc = ordered_den(h, h->den)) return rc;
   h->den_ready = true;
   h->den_exact = h->exact;
   h->den_ordered = ord;
   return TRACE_OK;
 }
 
-// Slices per thread of the ordered backprojector: up to 8, a divisor of the
-// slice-batched copy's batch for kOrdNum.
-static int ordered_batch(int mode, int S, int xsb) {
-  int sb = 8;
+// Slices per thread of the ordered backprojector: up to 32 in the tile form
+// and 8 in the per-pixel form, a divisor of the slice-batched copy's batch
+// for kOrdNum.
+static int ordered_batch(int mode, int S, int xsb, bool tile) {
+  int sb = tile ? 32 : 8;
   while (sb > 1 && sb / 2 >= S) sb /= 2;
   if (mode == kOrdNum)
     while (sb > 1 && (xsb % sb) != 0) sb /= 2;
   return sb;
 }
 
+int ordered_den(trace_handle *h, float *out) {
+  if (int rc = ensure_rays(h)) return rc;
+  if (h->ord_tile != 0 && !h->wraps) {
+    unsigned *ovf = reinterpret_cast<unsigned *>(h->flag);
+    HIP_TRY(hipMemsetAsync(ovf, 0, sizeof(unsigned), h->stream));
+    TIMED(TRACE_PHASE_SETUP, launch_bp_ordered(h->geo, h->rays, kOrdDen, h->exact, nullptr,
+                                               nullptr, 1, 1, out, false, ovf, h->stream));
+    unsigned f = 0;
+    HIP_TRY(hipMemcpyAsync(&f, ovf, sizeof(f), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->ord_tile = f ? 0 : 1;
+    if (!f) return TRACE_OK;
+  } else {
+    h->ord_tile = 0;
+  }
+  TIMED(TRACE_PHASE_SETUP, launch_bp_ordered(h->geo, h->rays, kOrdDen, h->exact, nullptr, nullptr,
+                                             1, 1, out, h->wraps, nullptr, h->stream));
+  return TRACE_OK;
+}
+
 int backproject_ordered(trace_handle *h, int mode, const float *w, float *z, int phase) {
   if (int rc = ensure_rays(h)) return rc;
-  const int sb = ordered_batch(mode, h->geo.S, h->xb_sb);
+  if (h->ord_tile < 0)  // decided by the den pass (same walks, same slots)
+    if (int rc = ordered_den(h, h->z)) return rc;
   if (mode == kOrdNum && !h->xb) return fail(TRACE_E_INVALID, "no slice-batched image copy");
+  const bool tile = h->ord_tile == 1;
+  const int sb = ordered_batch(mode, h->geo.S, h->xb_sb, tile);
+  // the tile form's overflow flag was checked on this geometry by the den pass
+  unsigned *ovf = tile ? reinterpret_cast<unsigned *>(h->flag) : nullptr;
   TIMED(phase, launch_bp_ordered(h->geo, h->rays, mode, h->exact, w, h->xb, h->xb_sb, sb, z,
-                                 h->wraps, h->stream));
+                                 h->wraps, ovf, h->stream));
   return TRACE_OK;
 }
 
@@ -152,7 +198,7 @@ int ensure_work(trace_handle *h, bool images, bool batched_copy) {
   const size_t nn = (size_t)h->geo.NN();
   const size_t S = (size_t)h->geo.S;
   if (h->xb && h->xb_sb != h->fwd_sb) {
-    (void)hipFree(h->xb);
+    dev_free(h->xb);
     h->xb = nullptr;
   }
   if (!h->xb && batched_copy) {
@@ -275,7 +321,11 @@ extern "C" {
 
 const char *trace_last_error(void) { return g_err.c_str(); }
 
-const char *trace_version(void) { return "trace-hip 0.2 (gfx950)"; }
+const char *trace_version(void) { return "trace-hip 0.3 (gfx950)"; }
+
+void trace_debug_fail_alloc(int after) { g_fail_after.store(after < 0 ? -1 : after); }
+
+int64_t trace_debug_live_allocations(void) { return (int64_t)g_live_allocs.load(); }
 
 void trace_deg2rad(const float *deg, float *rad, int n) {
   // trace_utils.cc:22: float product, double division, float result
@@ -388,7 +438,7 @@ void trace_destroy(trace_handle_t *h) {
   for (void *p : {(void *)h->geo.d_ang, (void *)h->geo.d_yi, (void *)h->rays, (void *)h->count,
                   (void *)h->den, (void *)h->xb, (void *)h->w, (void *)h->z, (void *)h->xa,
                   (void *)h->xc, (void *)h->flag, (void *)h->counters, (void *)h->maxw})
-    (void)hipFree(p);
+    dev_free(p);
   delete h;
 }
 
@@ -419,6 +469,11 @@ int trace_set_option(trace_handle_t *h, int option, int value) {
       if (value < 0 || value > 2) return fail(TRACE_E_INVALID, "value must be 0, 1 or 2");
       h->ordered = value;
       return TRACE_OK;
+    case TRACE_OPT_ORDERED_TILE:
+      if (value != 0 && value != 1) return fail(TRACE_E_INVALID, "value must be 0 or 1");
+      h->ord_tile = value ? -1 : 0;
+      h->den_ready = false;  // den is recomputed (same bits) by the chosen form
+      return TRACE_OK;
     case TRACE_OPT_FWD_LANES:
       if (value != 1 && value != 2 && value != 4)
         return fail(TRACE_E_INVALID, "lanes per ray must be 1, 2 or 4");
@@ -441,6 +496,7 @@ int trace_get_option(trace_handle_t *h, int option, int *value) {
     case TRACE_OPT_FWD_BUNDLE: *value = h->fwd_bundle; return TRACE_OK;
     case TRACE_OPT_FWD_LANES: *value = h->fwd_lanes; return TRACE_OK;
     case TRACE_OPT_ORDERED_SUMS: *value = h->ordered; return TRACE_OK;
+    case TRACE_OPT_ORDERED_TILE: *value = h->ord_tile != 0 ? 1 : 0; return TRACE_OK;
     default: return fail(TRACE_E_UNSUPPORTED, "unknown option");
   }
 }
@@ -563,7 +619,7 @@ int trace_reconstruct_host(trace_handle_t *h, trace_alg_t alg, int iters,
   hipError_t e = dev_alloc(&ds, nsino);
   if (e == hipSuccess) e = dev_alloc(&dr, nrec);
   if (e != hipSuccess) {
-    (void)hipFree(ds);
+    dev_free(ds);
     return fail(e == hipErrorOutOfMemory ? TRACE_E_NOMEM : TRACE_E_HIP,
                 std::string("allocation: ") + hipGetErrorString(e));
   }
@@ -579,8 +635,8 @@ int trace_reconstruct_host(trace_handle_t *h, trace_alg_t alg, int iters,
   } else {
     (void)hipStreamSynchronize(h->stream);
   }
-  (void)hipFree(ds);
-  (void)hipFree(dr);
+  dev_free(ds);
+  dev_free(dr);
   return rc;
 }
 
@@ -618,10 +674,7 @@ int trace_column_sums(trace_handle_t *h, float *den) {
   if (!den) return fail(TRACE_E_INVALID, "null buffer");
   HIP_TRY(hipSetDevice(h->device));
   if (h->ordered == 0) return backproject(h, kBpOutOnes, nullptr, den, TRACE_PHASE_SETUP);
-  if (int rc = ensure_rays(h)) return rc;
-  TIMED(TRACE_PHASE_SETUP, launch_bp_ordered(h->geo, h->rays, kOrdDen, h->exact, nullptr,
-                                             nullptr, 1, 1, den, h->wraps, h->stream));
-  return TRACE_OK;
+  return ordered_den(h, den);
 }
 
 int trace_segment_counts(trace_handle_t *h, float *counts) {

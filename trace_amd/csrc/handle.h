@@ -29,9 +29,16 @@ void clear_error();
     }                                                                        \
   } while (0)
 
+// Every device allocation of the library goes through dev_alloc and is
+// released by dev_free: they keep a process-wide count of live allocations
+// and can be told to fail once (trace_debug_fail_alloc), so that tests reach
+// the out-of-memory paths from the host.
+hipError_t device_alloc(void **p, size_t bytes);
+void dev_free(void *p);
+
 template <typename T>
 hipError_t dev_alloc(T **p, size_t count) {
-  return hipMalloc((void **)p, count * sizeof(T));
+  return device_alloc(reinterpret_cast<void **>(p), count * sizeof(T));
 }
 
 // HIP-event brackets around kernel launches, per phase.
@@ -111,6 +118,7 @@ struct trace_handle {
   bool den_exact = true;
   bool den_ordered = false;    // den summed in the reference's order
   int ordered = 1;             // TRACE_OPT_ORDERED_SUMS
+  int ord_tile = -1;           // tile form of the ordered sums: -1 untried, 1 usable, 0 not
   float *xb = nullptr;         // [ceil(S/fwd_sb)][N][N][fwd_sb]
   int xb_sb = 0;
   float *w = nullptr;          // [P][C][S] per-ray weights
@@ -154,6 +162,10 @@ int ensure_den(trace_handle *h);
 int ensure_work(trace_handle *h, bool images, bool batched_copy = true);
 // A^T applied with the configured backprojector (see capi.hip).
 int backproject(trace_handle *h, int mode, const float *w, float *z, int phase);
+// den = A^T 1 summed in the reference's order into out [N][N]: the tile
+// form where the geometry allows it (decided once, synchronously), else
+// the per-pixel form.
+int ordered_den(trace_handle *h, float *out);
 // Backprojected sums of w in the reference's order (kOrdNum: PML/APMLR num
 // with the x factor from the slice-batched copy xb; kOrdEM: SIRT/MLEM num).
 int backproject_ordered(trace_handle *h, int mode, const float *w, float *z, int phase);

@@ -10,6 +10,18 @@
 
 namespace trace_hip {
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b to XCD b % 8),
+// and each XCD has its own L2.  Renumbering so that XCD x runs a contiguous
+// range of logical blocks puts neighbouring work (adjacent columns of one
+// angle group, adjacent tiles) behind one L2: measured 2% on the forward
+// projector at 2048^2 / 1500 angles.  A bijection of [0, nb).
+__device__ __forceinline__ int64_t xcd_chunked(int64_t b, int64_t nb) {
+  constexpr int kXcd = 8;
+  const int64_t q = nb / kXcd, r = nb % kXcd;
+  const int64_t x = b % kXcd, j = b / kXcd;
+  return x * q + (x < r ? x : r) + j;
+}
+
 enum Alg { kSIRT = 0, kMLEM = 1, kPML = 2, kAPMLR = 3 };
 enum FwdModeHost { kFwdOutY = 0, kFwdOutSIRT = 1, kFwdOutEM = 2 };
 enum BpModeHost { kBpOutSino = 0, kBpOutOnes = 1, kBpOutCount = 2 };
@@ -115,10 +127,13 @@ struct BatchQueues {
 // Backprojection in the reference's summation order (ordered.hip): w
 // [P][C][S] weights, xb the slice-batched image the ray pass read (xsb
 // slices per slot batch; kOrdNum only), sb slices per thread (a divisor of
-// xsb for kOrdNum).
+// xsb for kOrdNum; at most 8 in the per-pixel form).  With tile_overflow
+// non-null the tile form runs (geometries without wrapped segments) and
+// ORs a nonzero value into *tile_overflow when a pixel needs more LDS
+// slots than it has; without, the per-pixel form.
 hipError_t launch_bp_ordered(const Geometry &g, const RayInfo *rays, int mode, bool exact,
                              const float *w, const float *xb, int xsb, int sb, float *out,
-                             bool wraps, hipStream_t st);
+                             bool wraps, unsigned *tile_overflow, hipStream_t st);
 bool ordered_sb_supported(int mode, int sb, int xsb);
 
 hipError_t launch_backproject(const Geometry &g, const RayInfo *rays, const int *count,

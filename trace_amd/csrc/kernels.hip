@@ -37,17 +37,6 @@ namespace {
 
 constexpr int kRayBlock = 256;
 
-// Workgroups are dealt to the 8 XCDs round-robin (workgroup b to XCD b % 8),
-// and each XCD has its own L2.  Renumbering so that XCD x runs a contiguous
-// range of logical blocks puts neighbouring work (adjacent columns of one
-// angle group, adjacent tiles) behind one L2: measured 2% on the forward
-// projector at 2048^2 / 1500 angles.  A bijection of [0, nb).
-__device__ __forceinline__ int64_t xcd_chunked(int64_t b, int64_t nb) {
-  constexpr int kXcd = 8;
-  const int64_t q = nb / kXcd, r = nb % kXcd;
-  const int64_t x = b % kXcd, j = b / kXcd;
-  return x * q + (x < r ? x : r) + j;
-}
 constexpr int kPixBlock = 256;
 constexpr int kTileBlock = 256;
 

@@ -29,6 +29,21 @@
 // 0 of the next row, trace_utils.cc:177-185) belong to strips at column 0.
 // Such strips walk the flagged rays (kRayWraps) crossing the strip left of
 // the right edge whole, so that their segments keep the ray's order.
+//
+// Tile form (k_bp_ordered_tile, the default where it applies): a wavefront
+// owns an 8 x 8 tile and takes the projections four at a time.  Its 64
+// lanes first walk the four projections' columns through the tile at once
+// (one (projection, column) pair per lane, as the LDS tile backprojector
+// does) and drop every segment's length into an LDS slot keyed by (pixel,
+// projection, column - the pixel's first candidate column, segment number
+// of that ray in that pixel); then each lane sums its own pixel's slots in
+// (projection, column, segment) order.  A pixel sees at most three
+// candidate columns per projection (its projected width is at most sqrt 2
+// plus twice the 0.45-pixel margin), and a ray puts at most two segments in
+// one pixel on the geometries measured (two where it grazes a corner); a
+// geometry that would need more slots sets a flag when den is computed, and
+// the handle then keeps the per-pixel form.  Geometries with wrapped
+// segments use the per-pixel form.
 #include <hip/hip_runtime.h>
 
 #include "internal.h"
@@ -185,17 +200,170 @@ hipError_t ord_launch(const OrdArgs &a, bool exact, hipStream_t st) {
   return hipGetLastError();
 }
 
+
+// Tile form: see the header.  One wavefront per (8 x 8 tile, slice batch).
+constexpr int kOT = 8;       // tile side
+constexpr int kOA = 4;       // projections per round
+constexpr int kORanks = 16;  // column slots per projection (>= the tile's count bound)
+constexpr int kOSlots = 3;   // candidate columns per pixel and projection
+constexpr int kODups = 2;    // segments per ray and pixel
+constexpr int kOWave = 64;
+
+template <int SB>
+struct OrdTileLds {
+  static constexpr int kWPitch = SB >= 4 ? SB + 4 : SB;  // row pitch of s_w (bank spread)
+};
+
+template <int SB, int MODE, bool kExact>
+__global__ void __launch_bounds__(kOWave)
+k_bp_ordered_tile(OrdArgs a, unsigned *__restrict__ overflow) {
+  constexpr int WP = OrdTileLds<SB>::kWPitch;
+  __shared__ unsigned s_cnt[kOA][kOWave];                  // byte counter per column slot
+  __shared__ float s_len[kOA][kOSlots * kODups][kOWave];
+  __shared__ float s_w[MODE == kOrdDen ? 1 : kOA * kORanks * WP];
+  __shared__ int s_cmin[kOA];
+  const int N = a.g.N, C = a.g.C;
+  const int tiles_x = (N + kOT - 1) / kOT;
+  const int tile = (int)xcd_chunked(blockIdx.x, gridDim.x);
+  const int tx0 = (tile % tiles_x) * kOT, ty0 = (tile / tiles_x) * kOT;
+  const int tx1 = min(tx0 + kOT, N), ty1 = min(ty0 + kOT, N);
+  const int b = blockIdx.y;
+  const int lane = threadIdx.x;
+  const int64_t NN = (int64_t)N * N;
+  // the pixel this lane sums
+  const int px = tx0 + (lane % kOT), py = ty0 + (lane / kOT);
+  const bool pix_ok = px < N && py < N;
+  const float pcx0 = grid_line(N, px), pcx1 = grid_line(N, px + 1);
+  const float pcy0 = grid_line(N, py), pcy1 = grid_line(N, py + 1);
+  float acc[SB], xv[SB];
+#pragma unroll
+  for (int q = 0; q < SB; ++q) acc[q] = 0.f, xv[q] = 0.f;
+  if (MODE == kOrdNum && pix_ok) {
+    const int s0 = b * SB;
+    const float *src =
+        a.xb + ((int64_t)(s0 / a.xsb) * NN + (int64_t)py * N + px) * a.xsb + s0 % a.xsb;
+#pragma unroll
+    for (int q = 0; q < SB; ++q) xv[q] = src[q];
+  }
+  const float x0 = grid_line(N, tx0), x1 = grid_line(N, tx1);
+  const float y0 = grid_line(N, ty0), y1 = grid_line(N, ty1);
+#pragma unroll
+  for (int ai = 0; ai < kOA; ++ai) s_cnt[ai][lane] = 0;
+  __syncthreads();
+  const int ai1 = lane / kORanks, k1 = lane % kORanks;  // phase-1 pair of this lane
+  for (int p0 = 0; p0 < a.P; p0 += kOA) {
+    // phase 1: walk the pairs, file each segment of the tile in its slot
+    const int p = p0 + ai1;
+    if (p < a.P) {
+      const AngleInfo ang = a.ang[p];
+      int cmin, cmax;
+      tile_columns(ang, x0, x1, y0, y1, a.yi0, cmin, cmax);
+      if (k1 == 0) s_cmin[ai1] = cmin;
+      const int c = cmin + k1;
+      if (c <= cmax && c >= 0 && c < C) {
+        const int64_t t = (int64_t)p * C + c;
+        if (MODE != kOrdDen) {
+          const float *wr = a.w + t * a.S + b * SB;
+          float *dst = s_w + (ai1 * kORanks + k1) * WP;
+#pragma unroll
+          for (int q = 0; q < SB; ++q) dst[q] = (b * SB + q < a.S) ? wr[q] : 0.f;
+        }
+        const RayInfo ray = a.rays[t];
+        traverse_window<kExact, true>(a.g, ray, ang.quadrant, tx0, tx1, ty0, ty1,
+                                      [&](int ix, int iy, float l, float) {
+          const unsigned lx = (unsigned)(ix - tx0), ly = (unsigned)(iy - ty0);
+          if (lx < (unsigned)(tx1 - tx0) && ly < (unsigned)(ty1 - ty0)) {
+            int pc0, pc1;
+            tile_columns(ang, grid_line(N, ix), grid_line(N, ix + 1), grid_line(N, iy),
+                         grid_line(N, iy + 1), a.yi0, pc0, pc1);
+            const int slot = c - pc0;
+            if ((unsigned)slot >= (unsigned)kOSlots) {
+              atomicOr(overflow, 1u);
+              return;
+            }
+            const int lp = (int)(ly * kOT + lx);
+            const unsigned old = atomicAdd(&s_cnt[ai1][lp], 1u << (8 * slot));
+            const int dup = (int)((old >> (8 * slot)) & 0xffu);
+            if (dup >= kODups) {
+              atomicOr(overflow, 2u);
+              return;
+            }
+            s_len[ai1][slot * kODups + dup][lp] = l;
+          }
+        });
+      }
+    }
+    __syncthreads();
+    // phase 2: each lane adds its pixel's segments in order
+    if (pix_ok) {
+#pragma unroll
+      for (int ai = 0; ai < kOA; ++ai) {
+        const unsigned word = s_cnt[ai][lane];
+        if (word == 0 || p0 + ai >= a.P) continue;
+        const AngleInfo ang = a.ang[p0 + ai];
+        int pc0, pc1;
+        tile_columns(ang, pcx0, pcx1, pcy0, pcy1, a.yi0, pc0, pc1);
+        const int kbase = pc0 - s_cmin[ai];
+#pragma unroll
+        for (int slot = 0; slot < kOSlots; ++slot) {
+          const int n = min((int)((word >> (8 * slot)) & 0xffu), kODups);
+          for (int d = 0; d < n; ++d) {
+            const float l = s_len[ai][slot * kODups + d][lane];
+            if (MODE == kOrdDen) {
+              acc[0] = acc[0] + l;
+            } else {
+              const float *wr = s_w + (ai * kORanks + kbase + slot) * WP;
+#pragma unroll
+              for (int q = 0; q < SB; ++q) {
+                if (MODE == kOrdEM)
+                  acc[q] = acc[q] + l * wr[q];
+                else
+                  acc[q] = acc[q] - (xv[q] * l) * wr[q];
+              }
+            }
+          }
+        }
+        s_cnt[ai][lane] = 0;
+      }
+    }
+    __syncthreads();
+  }
+  if (!pix_ok) return;
+  const int64_t j = (int64_t)py * N + px;
+  if (MODE == kOrdDen) {
+    a.out[j] = acc[0];
+    return;
+  }
+#pragma unroll
+  for (int q = 0; q < SB; ++q)
+    if (b * SB + q < a.S) a.out[(int64_t)(b * SB + q) * NN + j] = acc[q];
+}
+
+template <int SB, int MODE>
+hipError_t ord_tile_launch(const OrdArgs &a, bool exact, unsigned *overflow, hipStream_t st) {
+  const int N = a.g.N;
+  const int tiles = ((N + kOT - 1) / kOT) * ((N + kOT - 1) / kOT);
+  const int nb = MODE == kOrdDen ? 1 : (a.S + SB - 1) / SB;
+  if (exact)
+    hipLaunchKernelGGL((k_bp_ordered_tile<SB, MODE, true>), dim3(tiles, nb), dim3(kOWave), 0, st,
+                       a, overflow);
+  else
+    hipLaunchKernelGGL((k_bp_ordered_tile<SB, MODE, false>), dim3(tiles, nb), dim3(kOWave), 0,
+                       st, a, overflow);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 bool ordered_sb_supported(int mode, int sb, int xsb) {
   if (mode == kOrdDen) return true;
-  if (sb != 1 && sb != 2 && sb != 4 && sb != 8) return false;
+  if (sb != 1 && sb != 2 && sb != 4 && sb != 8 && sb != 16 && sb != 32) return false;
   return mode == kOrdEM || (xsb >= sb && xsb % sb == 0);
 }
 
 hipError_t launch_bp_ordered(const Geometry &g, const RayInfo *rays, int mode, bool exact,
                              const float *w, const float *xb, int xsb, int sb, float *out,
-                             bool wraps, hipStream_t st) {
+                             bool wraps, unsigned *tile_overflow, hipStream_t st) {
   OrdArgs a;
   a.g = g.grid;
   a.ang = g.d_ang;
@@ -209,6 +377,26 @@ hipError_t launch_bp_ordered(const Geometry &g, const RayInfo *rays, int mode, b
   a.out = out;
   a.wraps = wraps ? 1 : 0;
   if (!ordered_sb_supported(mode, sb, xsb)) return hipErrorInvalidValue;
+  if (tile_overflow) {
+    // tile form (no wrapped segments)
+    if (wraps) return hipErrorInvalidValue;
+    if (mode == kOrdDen) return ord_tile_launch<1, kOrdDen>(a, exact, tile_overflow, st);
+#define TRACE_ORDT_CASE(SB_)                                                               \
+  case SB_:                                                                                \
+    return mode == kOrdNum ? ord_tile_launch<SB_, kOrdNum>(a, exact, tile_overflow, st)    \
+                           : ord_tile_launch<SB_, kOrdEM>(a, exact, tile_overflow, st);
+    switch (sb) {
+      TRACE_ORDT_CASE(1)
+      TRACE_ORDT_CASE(2)
+      TRACE_ORDT_CASE(4)
+      TRACE_ORDT_CASE(8)
+      TRACE_ORDT_CASE(16)
+      TRACE_ORDT_CASE(32)
+      default: return hipErrorInvalidValue;
+    }
+#undef TRACE_ORDT_CASE
+  }
+  if (sb > 8) return hipErrorInvalidValue;
   if (mode == kOrdDen) return ord_launch<2, 2, 1, kOrdDen>(a, exact, st);
 #define TRACE_ORD_CASE(SB_)                                                              \
   case SB_:                                                                              \

@@ -15,6 +15,7 @@ Two levels, both over the C-ABI (``include/trace_hip.h``):
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as ct
 from dataclasses import dataclass
 
@@ -68,7 +69,15 @@ def _dptr(t) -> int:
 
 
 class Projector:
-    """Geometry of one slice stack on one GPU (a C-ABI handle)."""
+    """Geometry of one slice stack on one GPU (a C-ABI handle).
+
+    The handle's kernels run on the stream given at construction (by
+    default torch's current stream at that time).  A call made while torch's
+    current stream is another one is ordered against it: the handle's
+    stream first waits for the work queued so far on the current stream
+    (which produced the inputs), and the current stream then waits for the
+    call's kernels (which produce the outputs).  A handle on the null
+    stream called from another stream synchronises instead."""
 
     def __init__(self, theta_rad, num_slices: int, num_cols: int,
                  center: float = 0.0, device: int = 0, stream=None,
@@ -98,6 +107,34 @@ class Projector:
         self._h = h
         if not exact_sqrt:
             self.set_exact_sqrt(False)
+
+    @contextlib.contextmanager
+    def _ordered(self):
+        """Orders a call against torch's current stream (see the class).
+        A handle on the null stream is ordered by synchronising: the null
+        stream the library launches on need not be the stream torch means by
+        0, so events recorded through torch do not order it."""
+        import torch
+        cur = torch.cuda.current_stream(self.device)
+        if cur.cuda_stream == self.stream:
+            yield
+            return
+        if not self.stream:
+            cur.synchronize()
+            try:
+                yield
+            finally:
+                torch.cuda.synchronize(self.device)
+            return
+        hs = getattr(self, "_hstream", None)
+        if hs is None:
+            hs = torch.cuda.ExternalStream(self.stream, device=self.device)
+            self._hstream = hs
+        hs.wait_stream(cur)
+        try:
+            yield
+        finally:
+            cur.wait_stream(hs)
 
     # -- lifetime --------------------------------------------------------
     def close(self):
@@ -149,6 +186,10 @@ class Projector:
         reference's order; 2 also the SIRT/MLEM num."""
         check(lib().trace_set_option(self._h, _capi.TRACE_OPT_ORDERED_SUMS, int(level)))
 
+    def set_ordered_tile(self, on: bool):
+        """Tile (default) or per-pixel form of the ordered sums (same bits)."""
+        check(lib().trace_set_option(self._h, _capi.TRACE_OPT_ORDERED_TILE, 1 if on else 0))
+
     def get_option(self, option: int) -> int:
         v = ct.c_int()
         check(lib().trace_get_option(self._h, int(option), ct.byref(v)))
@@ -169,7 +210,8 @@ class Projector:
     def segment_counts(self, out=None):
         """Segments per pixel of one slice, [N][N] (float, integer-valued)."""
         d = self._empty(self.N, self.N) if out is None else out
-        check(lib().trace_segment_counts(self._h, _dptr(d)))
+        with self._ordered():
+            check(lib().trace_segment_counts(self._h, _dptr(d)))
         return d
 
     def set_phase_timing(self, on: bool):
@@ -191,26 +233,30 @@ class Projector:
     def forward(self, x, out=None):
         """y = A x; x [S][N][N] -> y [P][S][C]."""
         y = self._empty(self.P, self.S, self.C) if out is None else out
-        check(lib().trace_forward(self._h, _dptr(x), _dptr(y)))
+        with self._ordered():
+            check(lib().trace_forward(self._h, _dptr(x), _dptr(y)))
         return y
 
     def adjoint(self, r, out=None):
         """z = A^T r; r [P][S][C] -> z [S][N][N]."""
         z = self._empty(self.S, self.N, self.N) if out is None else out
-        check(lib().trace_adjoint(self._h, _dptr(r), _dptr(z)))
+        with self._ordered():
+            check(lib().trace_adjoint(self._h, _dptr(r), _dptr(z)))
         return z
 
     def column_sums(self, out=None):
         """A^T 1 for one slice, [N][N]."""
         d = self._empty(self.N, self.N) if out is None else out
-        check(lib().trace_column_sums(self._h, _dptr(d)))
+        with self._ordered():
+            check(lib().trace_column_sums(self._h, _dptr(d)))
         return d
 
     def ray_sums(self):
         """(sum len, sum len^2) per ray of one slice, each [P][C]."""
         a1 = self._empty(self.P, self.C)
         a2 = self._empty(self.P, self.C)
-        check(lib().trace_ray_sums(self._h, _dptr(a1), _dptr(a2)))
+        with self._ordered():
+            check(lib().trace_ray_sums(self._h, _dptr(a1), _dptr(a2)))
         return a1, a2
 
     def reconstruct(self, alg: str, iters: int, sino, recon,
@@ -219,8 +265,9 @@ class Projector:
         sino [P][S][C] and recon [S][N][N]."""
         a = ALGS[alg] if isinstance(alg, str) else int(alg)
         prm = (params or RegParams()).c()
-        check(lib().trace_reconstruct(self._h, a, int(iters), _dptr(sino),
-                                      _dptr(recon), ct.byref(prm)))
+        with self._ordered():
+            check(lib().trace_reconstruct(self._h, a, int(iters), _dptr(sino),
+                                          _dptr(recon), ct.byref(prm)))
         return recon
 
     def reconstruct_host(self, alg: str, iters: int, sino: np.ndarray,
