@@ -184,6 +184,19 @@ int trace_reconstruct_host(trace_handle_t *h, trace_alg_t alg, int iters,
  * runs in the reference's segment order (CalculateSimdata, sirt.cc:4-19). */
 int trace_forward(trace_handle_t *h, const float *x, float *y);
 
+/* y = A x as trace_forward, allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid it).  Its
+ * scratch -- the ray table and the slice-batched image copy, P*C*32 bytes
+ * (rounded up to 256) plus ceil(S/b)*b*N*N*4 bytes, b the forward slice
+ * batch of the launch shape -- lives in a static 128 MiB device array that
+ * every GPU the library runs on holds until the process ends; a geometry
+ * that needs more gets TRACE_E_INVALID before anything is launched.  All
+ * calls on one GPU share that array: calls that overlap on one GPU (other
+ * streams, other host threads) race.  Same result as trace_forward, bit for
+ * bit. */
+int trace_forward_noalloc(trace_handle_t *h, const float *x, float *y);
+
 /* z = A^T r: z [S][N][N] (overwritten) from r [P][S][C] (device buffers).
  * The scatter half of UpdateReconReplica (sirt.cc:35-64) with a free
  * per-ray weight. */

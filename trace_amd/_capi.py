@@ -115,6 +115,7 @@ _SIGS = {
     "trace_reconstruct_host": (ct.c_int, [_vp, ct.c_int, ct.c_int, _vp, _vp,
                                           ct.POINTER(TraceParams)]),
     "trace_forward": (ct.c_int, [_vp, _vp, _vp]),
+    "trace_forward_noalloc": (ct.c_int, [_vp, _vp, _vp]),
     "trace_adjoint": (ct.c_int, [_vp, _vp, _vp]),
     "trace_column_sums": (ct.c_int, [_vp, _vp]),
     "trace_ray_sums": (ct.c_int, [_vp, _vp, _vp]),

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -226,6 +227,40 @@ int check_finite(trace_handle *h, const float *v, int64_t n) {
 }
 
 }  // namespace trace_hip
+
+// Scratch of trace_forward_noalloc: the ray table and the slice-batched
+// image copy of one call, in device memory the library never allocates at
+// run time.  Every GPU the library runs on holds the array until the
+// process ends; calls on one GPU share it.  In host code `scratch` names a
+// host-side stand-in, never the device memory: its device address comes
+// from hipGetSymbolAddress, once per device (scratch_address).
+constexpr size_t kScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (current), looked up on the
+// device's first call.
+int scratch_address(int device, void **out) {
+  if (device < 0 || device >= kMaxDevices) return fail(TRACE_E_INVALID, "device index too large");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(TRACE_E_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                   hipGetErrorString(e));
+    g_scratch[device] = p;
+  }
+  *out = g_scratch[device];
+  return TRACE_OK;
+}
+
+}  // namespace
 
 namespace {
 
@@ -652,6 +687,31 @@ int trace_forward(trace_handle_t *h, const float *x, float *y) {
   TIMED(TRACE_PHASE_SETUP,
         launch_pack(x, g.S, 0, 0.f, g.NN(), nbatch(g.S, sb) * sb, sb, h->xb, h->stream));
   return forward(h, kFwdOutY, nullptr, y);
+}
+
+int trace_forward_noalloc(trace_handle_t *h, const float *x, float *y) {
+  g_err.clear();
+  if (!h) return fail(TRACE_E_INVALID, "handle is null");
+  if (!x || !y) return fail(TRACE_E_INVALID, "null buffer");
+  const Geometry &g = h->geo;
+  const int sb = h->fwd_sb;
+  const size_t rays_b = ((size_t)g.rays_per_slice() * sizeof(RayInfo) + 255) & ~size_t(255);
+  const size_t xb_b = (size_t)nbatch(g.S, sb) * sb * (size_t)g.NN() * sizeof(float);
+  if (rays_b + xb_b > kScratchBytes)
+    return fail(TRACE_E_INVALID, "this geometry needs " +
+                                     std::to_string((rays_b + xb_b + (1 << 20) - 1) >> 20) +
+                                     " MiB of scratch; trace_forward_noalloc has 128 MiB");
+  HIP_TRY(hipSetDevice(h->device));
+  void *base = nullptr;
+  if (int rc = scratch_address(h->device, &base)) return rc;
+  RayInfo *rays = static_cast<RayInfo *>(base);
+  float *xb = reinterpret_cast<float *>(static_cast<unsigned char *>(base) + rays_b);
+  // every scratch byte read below is written first on the same stream
+  HIP_TRY(launch_ray_info(g, rays, h->stream, false));
+  HIP_TRY(launch_pack(x, g.S, 0, 0.f, g.NN(), nbatch(g.S, sb) * sb, sb, xb, h->stream));
+  HIP_TRY(launch_forward(g, rays, sb, h->fwd_bundle, h->fwd_lanes, kFwdOutY, h->exact, xb,
+                         nullptr, y, nullptr, h->stream));
+  return TRACE_OK;
 }
 
 int trace_adjoint(trace_handle_t *h, const float *r, float *z) {

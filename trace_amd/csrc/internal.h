@@ -69,7 +69,10 @@ struct RegParams {
 int tile_count_bound(float sinq, float cosq, int TW, int TH);
 
 // Launchers (kernels.hip).  All are asynchronous on `st`.
-hipError_t launch_ray_info(const Geometry &g, RayInfo *rays, hipStream_t st);
+// Ray table; `flags` adds the window flags (kRayWhole, kRayWraps, kRay1D)
+// from one walk per ray, which the tile kernels need and whole-ray walks
+// do not.
+hipError_t launch_ray_info(const Geometry &g, RayInfo *rays, hipStream_t st, bool flags = true);
 hipError_t launch_forward(const Geometry &g, const RayInfo *rays, int sb, int bundle_angles,
                           int lanes_per_ray, int mode, bool exact, const float *xb,
                           const float *sino, float *out, unsigned *maxw, hipStream_t st);

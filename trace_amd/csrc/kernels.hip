@@ -48,12 +48,14 @@ __device__ __forceinline__ void atomic_add_f32(float *p, float v) {
 
 __global__ void __launch_bounds__(kRayBlock)
 k_ray_info(GridInfo g, const AngleInfo *__restrict__ ang, const float *__restrict__ yi,
-           int P, RayInfo *__restrict__ rays) {
+           int P, int flags, RayInfo *__restrict__ rays) {
   const int64_t t = (int64_t)blockIdx.x * kRayBlock + threadIdx.x;
   if (t >= (int64_t)P * g.C) return;
   const int c = (int)(t % g.C), p = (int)(t / g.C);
   RayInfo r = make_ray(g, yi[c], ang[p].sinq, ang[p].cosq);
-  r.alo |= ray_flags(g, r, ang[p].quadrant);
+  // the flags walk each ray once; whole-ray walks (the forward projector)
+  // do not read them
+  if (flags) r.alo |= ray_flags(g, r, ang[p].quadrant);
   rays[t] = r;
 }
 
@@ -1192,10 +1194,10 @@ int tile_count_bound(float sinq, float cosq, int TW, int TH) {
   return (int)std::floor(len) + 1;
 }
 
-hipError_t launch_ray_info(const Geometry &g, RayInfo *rays, hipStream_t st) {
+hipError_t launch_ray_info(const Geometry &g, RayInfo *rays, hipStream_t st, bool flags) {
   const int64_t n = g.rays_per_slice();
   hipLaunchKernelGGL(k_ray_info, dim3(blocks_for(n, kRayBlock)), dim3(kRayBlock), 0, st,
-                     g.grid, g.d_ang, g.d_yi, g.P, rays);
+                     g.grid, g.d_ang, g.d_yi, g.P, flags ? 1 : 0, rays);
   return hipGetLastError();
 }
 
