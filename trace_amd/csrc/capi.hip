@@ -305,8 +305,8 @@ int forward(trace_handle *h, int mode, const float *sino, float *out) {
     TIMED(TRACE_PHASE_SETUP,
           launch_fill(reinterpret_cast<float *>(h->maxw), h->geo.S, 0.f, h->stream));
   TIMED(TRACE_PHASE_FORWARD, launch_forward(h->geo, h->rays, h->fwd_sb, h->fwd_bundle,
-                                            h->fwd_lanes, mode,
-                                            h->exact, h->xb, sino, out, h->maxw, h->stream));
+                                            h->fwd_lanes, mode, h->exact, h->xb, sino, out,
+                                            h->maxw, h->stream));
   return TRACE_OK;
 }
 

@@ -371,6 +371,48 @@ __device__ __forceinline__ void bp_tile_loop(const TileArgs &a, BpWord<MODE> *ac
   // 2048^2 slice at 32x32 tiles; 532 vs 109 ms for 64 slices).
   constexpr bool kRank = (SB == 1);
   constexpr int kRanks = isqrt_ceil(TW * TW + TH * TH) + 4 + extra;  // >= count[p] + extra
+  // Slot iteration: (r, p, k) -> the lane's next (angle, column slot).
+  auto advance = [&](int &r, int &p, int &k) {
+    if (kRank) {
+      p += kTileBlock;
+      while (p >= a.P) { p -= a.P; ++r; }
+    } else {
+      k += kTileBlock;
+      while (p < a.P && k >= count[p] + extra) { k -= count[p] + extra; ++p; }
+    }
+  };
+  auto live = [&](int r, int p) { return kRank ? r < kRanks : p < a.P; };
+  // One (angle, column) pair, fetched a pair ahead of its walk: the ray
+  // table entry and the weights mostly miss L2, so their loads for the next
+  // pair are in flight while the current pair is walked.
+  struct Pair {
+    bool ok;
+    int quadrant;
+    RayInfo ray;
+    float w[SB];
+  };
+  auto fetch = [&](int r, int p, int k, Pair &q) {
+    q.ok = false;
+    if (!live(r, p)) return;
+    if (kRank) {
+      const int cnt = count[p] + extra;
+      if (r >= cnt) return;
+      k = ((cnt - 1) >> 1) + ((r & 1) ? (r + 1) >> 1 : -(r >> 1));
+    }
+    int cmin, cmax;
+    const AngleInfo ang = a.ang[p];
+    tile_columns(ang, x0, x1, y0, y1, a.yi0, cmin, cmax);
+    const int c = cmin + k;
+    if (c < 0 || c >= C || c > cmax) return;
+    q.ok = true;
+    q.quadrant = ang.quadrant;
+    q.ray = a.rays[(int64_t)p * C + c];
+    if (MODE == kBpSino) {
+      const float *wr = a.w + ((int64_t)p * C + c) * a.S + b * SB;
+#pragma unroll
+      for (int i = 0; i < SB; ++i) q.w[i] = (b * SB + i < a.S) ? wr[i] : 0.f;
+    }
+  };
   int r = 0, p = 0, k = threadIdx.x;
   if (kRank) {
     r = threadIdx.x / a.P;
@@ -378,40 +420,30 @@ __device__ __forceinline__ void bp_tile_loop(const TileArgs &a, BpWord<MODE> *ac
   } else {
     while (p < a.P && k >= count[p] + extra) { k -= count[p] + extra; ++p; }
   }
-  while (kRank ? r < kRanks : p < a.P) {
-    if (kRank) {
-      const int cnt = count[p] + extra;
-      if (r >= cnt) {
-        p += kTileBlock;
-        while (p >= a.P) { p -= a.P; ++r; }
-        continue;
-      }
-      k = ((cnt - 1) >> 1) + ((r & 1) ? (r + 1) >> 1 : -(r >> 1));
-    }
-    int cmin, cmax;
-    const AngleInfo ang = a.ang[p];
-    tile_columns(ang, x0, x1, y0, y1, a.yi0, cmin, cmax);
-    const int c = cmin + k;
-    if (c >= 0 && c < C && c <= cmax) {
+  Pair cur;
+  fetch(r, p, k, cur);
+  while (live(r, p)) {
+    int rn = r, pn = p, kn = k;
+    advance(rn, pn, kn);
+    Pair nxt;
+    fetch(rn, pn, kn, nxt);
+    if (cur.ok) {
       // weights pre-scaled by 2^e (exact: a power-of-two scaling commutes
       // with rounding), so fl(len * ws) = fl(len * w) * 2^e per segment
       float wv[SB];
       bool any = true;
       if (MODE == kBpSino) {
         any = false;
-        const float *wr = a.w + ((int64_t)p * C + c) * a.S + b * SB;
 #pragma unroll
         for (int q = 0; q < SB; ++q) {
-          const float wq = (b * SB + q < a.S) ? wr[q] : 0.f;
-          any |= (wq != 0.f);
-          wv[q] = (ALLFX || fx[q]) ? ldexpf(wq, ex[q]) : wq;
+          any |= (cur.w[q] != 0.f);
+          wv[q] = (ALLFX || fx[q]) ? ldexpf(cur.w[q], ex[q]) : cur.w[q];
         }
       } else if (MODE == kBpOnes) {
         wv[0] = ldexpf(1.0f, ex[0]);
       }
-      const RayInfo ray = a.rays[(int64_t)p * C + c];
-      if (any && (!EDGE || (ray.alo & kRayWraps))) {
-        traverse_window<kExact, kRank>(a.g, ray, ang.quadrant, wx0, wx1, wy0, wy1,
+      if (any && (!EDGE || (cur.ray.alo & kRayWraps))) {
+        traverse_window<kExact, kRank>(a.g, cur.ray, cur.quadrant, wx0, wx1, wy0, wy1,
                                 [&](int ix, int iy, float l, float) {
           // ix == N is pixel 0 of the next row (truncation); this pass takes
           // the wrapped segments (EDGE) or the others (the bounds test below
@@ -468,13 +500,10 @@ __device__ __forceinline__ void bp_tile_loop(const TileArgs &a, BpWord<MODE> *ac
         });
       }
     }
-    if (kRank) {
-      p += kTileBlock;
-      while (p >= a.P) { p -= a.P; ++r; }
-    } else {
-      k += kTileBlock;
-      while (p < a.P && k >= count[p] + extra) { k -= count[p] + extra; ++p; }
-    }
+    r = rn;
+    p = pn;
+    k = kn;
+    cur = nxt;
   }
 }
 
