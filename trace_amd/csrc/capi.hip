@@ -151,8 +151,9 @@ int ensure_den(trace_handle *h) {
 }
 
 // Slices per thread of the ordered backprojector: up to 32 in the tile form
-// and 8 in the per-pixel form, a divisor of the slice-batched copy's batch
-// for kOrdNum.
+// (16 for kOrdNum, whose per-pixel image values also take registers) and 8
+// in the per-pixel form, a divisor of the slice-batched copy's batch for
+// kOrdNum.
 static int ordered_batch(int mode, int S, int xsb, bool tile) {
   int sb = tile ? 32 : 8;
   while (sb > 1 && sb / 2 >= S) sb /= 2;

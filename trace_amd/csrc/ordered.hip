@@ -251,25 +251,51 @@ k_bp_ordered_tile(OrdArgs a, unsigned *__restrict__ overflow) {
   for (int ai = 0; ai < kOA; ++ai) s_cnt[ai][lane] = 0;
   __syncthreads();
   const int ai1 = lane / kORanks, k1 = lane % kORanks;  // phase-1 pair of this lane
+  // The lane's phase-1 pair of a round, fetched one round ahead: its ray
+  // table entry and weights mostly miss L2, so the next round's loads are
+  // issued before this round's phase 2 and arrive while it runs.
+  struct Pair {
+    int c, cmin, quadrant;
+    bool ok;
+    RayInfo ray;
+    float w[SB];
+  };
+  auto fetch = [&](int p, Pair &q) {
+    q.ok = false;
+    q.cmin = 0;
+    if (p >= a.P) return;
+    const AngleInfo ang = a.ang[p];
+    int cmin, cmax;
+    tile_columns(ang, x0, x1, y0, y1, a.yi0, cmin, cmax);
+    q.cmin = cmin;
+    q.quadrant = ang.quadrant;
+    q.c = cmin + k1;
+    if (q.c > cmax || q.c < 0 || q.c >= C) return;
+    q.ok = true;
+    const int64_t t = (int64_t)p * C + q.c;
+    q.ray = a.rays[t];
+    if (MODE != kOrdDen) {
+      const float *wr = a.w + t * a.S + b * SB;
+#pragma unroll
+      for (int i = 0; i < SB; ++i) q.w[i] = (b * SB + i < a.S) ? wr[i] : 0.f;
+    }
+  };
+  Pair cur;
+  fetch(ai1, cur);
   for (int p0 = 0; p0 < a.P; p0 += kOA) {
     // phase 1: walk the pairs, file each segment of the tile in its slot
     const int p = p0 + ai1;
     if (p < a.P) {
-      const AngleInfo ang = a.ang[p];
-      int cmin, cmax;
-      tile_columns(ang, x0, x1, y0, y1, a.yi0, cmin, cmax);
-      if (k1 == 0) s_cmin[ai1] = cmin;
-      const int c = cmin + k1;
-      if (c <= cmax && c >= 0 && c < C) {
-        const int64_t t = (int64_t)p * C + c;
+      if (k1 == 0) s_cmin[ai1] = cur.cmin;
+      if (cur.ok) {
+        const int c = cur.c;
         if (MODE != kOrdDen) {
-          const float *wr = a.w + t * a.S + b * SB;
           float *dst = s_w + (ai1 * kORanks + k1) * WP;
 #pragma unroll
-          for (int q = 0; q < SB; ++q) dst[q] = (b * SB + q < a.S) ? wr[q] : 0.f;
+          for (int q = 0; q < SB; ++q) dst[q] = cur.w[q];
         }
-        const RayInfo ray = a.rays[t];
-        traverse_window<kExact, true>(a.g, ray, ang.quadrant, tx0, tx1, ty0, ty1,
+        const AngleInfo ang = a.ang[p];
+        traverse_window<kExact, true>(a.g, cur.ray, cur.quadrant, tx0, tx1, ty0, ty1,
                                       [&](int ix, int iy, float l, float) {
           const unsigned lx = (unsigned)(ix - tx0), ly = (unsigned)(iy - ty0);
           if (lx < (unsigned)(tx1 - tx0) && ly < (unsigned)(ty1 - ty0)) {
@@ -294,6 +320,7 @@ k_bp_ordered_tile(OrdArgs a, unsigned *__restrict__ overflow) {
       }
     }
     __syncthreads();
+    fetch(p0 + kOA + ai1, cur);  // next round's pair, in flight during phase 2
     // phase 2: each lane adds its pixel's segments in order
     if (pix_ok) {
 #pragma unroll
