@@ -346,6 +346,8 @@ def run_stack(args):
 
     finite = bool(torch.isfinite(recon).all().item())
     nonneg = bool((recon >= 0).all().item()) if args.alg == "mlem" else None
+    invariants = stack_invariants(pj, sino, recon, args.alg, dev)
+    log(f"invariants: {invariants}")
 
     R = P * N
     fb, tw, th, tb = pj.launch_shape()
@@ -450,9 +452,37 @@ def run_stack(args):
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
         "batched": batched,
-        "checks": {"finite": finite, "nonnegative": nonneg},
+        "checks": {"finite": finite, "nonnegative": nonneg, **invariants},
     }
     print(json.dumps(out), flush=True)
+
+
+def stack_invariants(pj, sino, recon, alg, dev):
+    """Checks of the timed stack that need no reference run, at the launch
+    shape that was timed:
+      dot_product_rel  |<A u, v> - <u, A^T v>| / |<A u, v>| for random u, v
+                       on the whole stack (fp64 sums);
+      mlem_conservation_rel  after an MLEM iteration sum_j den_j x'_j =
+                       sum_r p_r per slice (x' = x A^T(p/Ax)/den, so
+                       sum_j den_j x'_j = sum_r p_r (Ax)_r/(Ax)_r): the largest
+                       relative violation over the slices."""
+    import torch
+    S, N = recon.shape[0], recon.shape[1]
+    g = torch.Generator(device=dev).manual_seed(5)
+    u = torch.rand(recon.shape, device=dev, generator=g)
+    v = torch.randn(sino.shape, device=dev, generator=g)
+    au = pj.forward(u)
+    atv = pj.adjoint(v)
+    lhs = (au.double() * v.double()).sum().item()
+    rhs = (u.double() * atv.double()).sum().item()
+    out = {"dot_product_rel": abs(lhs - rhs) / max(abs(lhs), 1e-300)}
+    del u, v, au, atv
+    if alg == "mlem":
+        den = pj.column_sums().double()
+        lhs = (recon.double() * den).sum(dim=(1, 2))
+        rhs = sino.double().sum(dim=(0, 2))
+        out["mlem_conservation_rel"] = float(((lhs - rhs).abs() / rhs.abs()).max().item())
+    return out
 
 
 def run_regularised(args):

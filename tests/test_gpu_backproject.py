@@ -193,3 +193,36 @@ def test_forward_lanes_rejects_uncompiled(gpu):
         pj.set_launch_shape(4, 16, 16, 4)
         with pytest.raises(TraceUnsupported):
             pj.set_fwd_lanes(2)  # batch 4 has no 2-lane form
+
+
+def test_bench_launch_shape_full_batches(gpu, oracle):
+    """The launch shape bench.py runs (forward: 32 slices per ray batch, 4
+    lanes per ray, 16-angle bundles; tiles 16x16 with 32 slices) on a stack
+    that fills two whole 32-slice batches: forward bit-exact, adjoint within
+    1e-6 of the oracle's fp64 adjoint, and two MLEM iterations within 1e-4
+    of the oracle."""
+    torch = gpu
+    from trace_amd import Projector
+    from trace_amd.autotune import apply_shape
+    from trace_amd.phantom import angles_rad, noisy_sinogram
+    N, P, S = 128, 90, 64
+    th = angles_rad(P)
+    rng = np.random.default_rng(11)
+    x = rng.uniform(0, 1, (S, N, N)).astype(np.float32)
+    r = rng.normal(size=(P, S, N)).astype(np.float32)
+    sino = noisy_sinogram(th, N, S, seed=9, I0=1e4, K=6).numpy()
+    print(f"bench shape N={N} P={P} S={S}", flush=True)
+    with Projector(th, S, N, N / 2.0) as pj:
+        apply_shape(pj, {"shape": [32, 16, 16, 32], "lanes": 4, "bundle": 16})
+        assert pj.launch_shape() == (32, 16, 16, 32)
+        assert pj.fwd_lanes() == 4 and pj.fwd_bundle() == 16
+        y = pj.forward(torch.from_numpy(x).cuda()).cpu().numpy()
+        z = pj.adjoint(torch.from_numpy(r).cuda()).cpu().numpy()
+        xr = torch.ones((S, N, N), device="cuda")
+        pj.reconstruct("mlem", 2, torch.from_numpy(sino).cuda(), xr)
+        torch.cuda.synchronize()
+        xr = xr.cpu().numpy()
+    assert np.array_equal(y.view(np.int32), oracle.forward(th, x, N / 2.0).view(np.int32))
+    assert rel_l2(z, oracle.adjoint(th, r, N / 2.0)) < 1e-6
+    xo = oracle.reconstruct("mlem", th, sino, np.ones((S, N, N), np.float32), N / 2.0, 2)
+    assert rel_l2(xr, xo) < 1e-4
