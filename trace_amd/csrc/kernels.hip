@@ -145,13 +145,22 @@ __device__ __forceinline__ void forward_bundle(const GridInfo &g, const AngleInf
     const RayInfo r = rays[t];
     traverse_full<kExact>(g, r, ang[p].quadrant, [&](int ix, int iy, float l, float l2) {
       const unsigned idx = (unsigned)(ix + iy * g.N);
-      if (idx < NN) {
+      const bool in = idx < NN;
+      if constexpr (SBL == 1) {
+        // branch-free: a segment outside the image (skipped) loads pixel 0
+        // and adds +0, which leaves the sum's bits unchanged (a sum that
+        // starts at +0 is never -0 in round-to-nearest)
+        float v[1];
+        load_slices<1>(xbb + (int64_t)(in ? idx : 0u) * SB, v);
+        const float t = v[0] * l;
+        acc[0] += in ? t : 0.f;
+      } else if (in) {
         float v[SBL];
         load_slices<SBL>(xbb + (int64_t)idx * SB, v);
 #pragma unroll
         for (int k = 0; k < SBL; ++k) acc[k] += v[k] * l;
-        hit = true;
       }
+      hit |= in;
       if (MODE == kFwdSIRT) a2 += l2;
     });
   }

@@ -248,39 +248,53 @@ __device__ __forceinline__ void segment(float px, float py, float qx, float qy, 
   iy = (int)(my + mg);
 }
 
-// Runs the merge from the walker's state, with an optional previous point,
-// until (i >= i_stop && j >= j_stop); visit(ix, iy, len, len2) per segment.
+// Takes `pts` merged points from the walker's state, with an optional
+// previous point; visit(ix, iy, len, len2) per segment.  Counted loop.
 template <bool kExactSqrt, typename Visit>
-__device__ __forceinline__ int walk(Walker &w, float mg, bool has_prev, float px, float py,
-                                    int i_stop, int j_stop, Visit &&visit) {
-  int nseg = 0;
-  const int limit = w.alen + w.blen;
-  for (int k = 0; k < limit; ++k) {
-    if (w.i >= i_stop && w.j >= j_stop) break;
+__device__ __forceinline__ int walk_points(Walker &w, float mg, bool has_prev, float px,
+                                           float py, int pts, Visit &&visit) {
+  if (pts <= 0) return 0;
+  if (!has_prev) {
+    w.next(px, py);
+    --pts;
+  }
+  for (int k = 0; k < pts; ++k) {
     float qx, qy;
     w.next(qx, qy);
-    if (has_prev) {
-      int ix, iy;
-      float l, l2;
-      segment<kExactSqrt>(px, py, qx, qy, mg, ix, iy, l, l2);
-      visit(ix, iy, l, l2);
-      ++nseg;
-    }
-    has_prev = true;
+    int ix, iy;
+    float l, l2;
+    segment<kExactSqrt>(px, py, qx, qy, mg, ix, iy, l, l2);
+    visit(ix, iy, l, l2);
     px = qx;
     py = qy;
   }
-  return nseg;
+  return pts;
 }
 
-// Whole ray in the reference's order.
+// Whole ray in the reference's order.  A whole walk takes exactly
+// alen + blen merged points (each step takes one of either list), so the
+// loop is counted, with the first point peeled: no per-step branch.
 template <bool kExactSqrt, typename Visit>
 __device__ __forceinline__ int traverse_full(const GridInfo &g, const RayInfo &r, int quadrant,
                                              Visit &&visit) {
   Walker w;
   w.init(r, g.N, quadrant);
   w.seek(0, 0);
-  return walk<kExactSqrt>(w, g.mg, false, 0.f, 0.f, w.alen, w.blen, visit);
+  const int n = w.alen + w.blen;
+  if (n < 2) return 0;
+  float px, py;
+  w.next(px, py);
+  for (int k = 1; k < n; ++k) {
+    float qx, qy;
+    w.next(qx, qy);
+    int ix, iy;
+    float l, l2;
+    segment<kExactSqrt>(px, py, qx, qy, g.mg, ix, iy, l, l2);
+    visit(ix, iy, l, l2);
+    px = qx;
+    py = qy;
+  }
+  return n - 1;
 }
 
 // The window consistency check of ray_flags on one axis (on_a: rows, where
@@ -395,7 +409,7 @@ __device__ __forceinline__ int traverse_window(const GridInfo &g, const RayInfo 
   w.init(r, g.N, quadrant);
   if (!w.a_sorted() || w.whole) {
     w.seek(0, 0);
-    return walk<kExactSqrt>(w, g.mg, false, 0.f, 0.f, w.alen, w.blen, visit);
+    return walk_points<kExactSqrt>(w, g.mg, false, 0.f, 0.f, w.alen + w.blen, visit);
   }
   const int M = kWindowMargin;
   const bool on_a = __builtin_fabsf(r.islope) <= 1.0f;
@@ -440,20 +454,25 @@ __device__ __forceinline__ int traverse_window(const GridInfo &g, const RayInfo 
     has_prev = true;
   }
   // Stop after the earlier of A[iE-1] and B[jE-1] (a window that reaches the
-  // end of its list does not bound the walk).
+  // end of its list does not bound the walk).  The merged position of the
+  // stop element follows from the same exact counts as the start: A[iE-1]
+  // comes after the count_b_le(its x) B elements, B[jE-1] after the
+  // count_a_lt(its x) A elements.  So the walk takes a known number of
+  // points, and its loop is counted (no per-step exit test).
   const bool endA = useA && iE < w.alen, endB = useB && jE < w.blen;
   const float ea = w.cross_a(w.a_line(endA ? iE - 1 : 0));
   const float eb = grid_line(g.N, w.blo + (endB ? jE - 1 : 0));
-  int i_stop = w.alen, j_stop = w.blen;
+  int i_end = w.alen, j_end = w.blen;
   if (endA && (!endB || ea < eb)) {
-    i_stop = iE;
-    j_stop = 0;
+    i_end = iE;
+    j_end = w.count_b_le(ea);
   } else if (endB) {
-    i_stop = 0;
-    j_stop = jE;
+    i_end = w.count_a_lt(eb);
+    j_end = jE;
   }
+  const int pts = min((i_end - i0) + (j_end - j0), w.alen + w.blen - i0 - j0);
   w.seek(i0, j0);
-  return walk<kExactSqrt>(w, g.mg, has_prev, px, py, i_stop, j_stop, visit);
+  return walk_points<kExactSqrt>(w, g.mg, has_prev, px, py, pts, visit);
 }
 
 // Columns whose ray can touch tile [x0,x1]x[y0,y1] at angle p: the ray of
