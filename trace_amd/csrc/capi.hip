@@ -207,7 +207,7 @@ int ensure_work(trace_handle *h, bool images, bool batched_copy) {
     HIP_TRY(dev_alloc(&h->xb, (size_t)nbatch((int)S, h->fwd_sb) * h->fwd_sb * nn));
     h->xb_sb = h->fwd_sb;
   }
-  if (!h->w) HIP_TRY(dev_alloc(&h->w, (size_t)h->geo.P * S * h->geo.C));
+  if (!h->w) HIP_TRY(dev_alloc(&h->w, (size_t)h->geo.P * S * h->geo.C + kWeightPad));
   if (!h->maxw) HIP_TRY(dev_alloc(&h->maxw, S));
   if (!h->z) HIP_TRY(dev_alloc(&h->z, S * nn));
   if (images) {

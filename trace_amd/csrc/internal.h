@@ -22,6 +22,16 @@ __device__ __forceinline__ int64_t xcd_chunked(int64_t b, int64_t nb) {
   return x * q + (x < r ? x : r) + j;
 }
 
+// s_waitcnt vmcnt(0): waits until every vector memory load of the wave has
+// returned (gfx9 encoding: vmcnt in bits 3:0 and 15:14; expcnt, bits 6:4,
+// and lgkmcnt, bits 11:8, at their maxima, i.e. not waited for).
+__device__ __forceinline__ void wait_vector_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Floats allocated after the [P][C][S] weights: the ordered tile kernel
+// reads a whole slice batch (up to 32 floats) from every row, so the last
+// row's partial batch stays inside the allocation.
+constexpr int kWeightPad = 32;
+
 enum Alg { kSIRT = 0, kMLEM = 1, kPML = 2, kAPMLR = 3 };
 enum FwdModeHost { kFwdOutY = 0, kFwdOutSIRT = 1, kFwdOutEM = 2 };
 enum BpModeHost { kBpOutSino = 0, kBpOutOnes = 1, kBpOutCount = 2 };

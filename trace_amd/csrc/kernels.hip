@@ -355,6 +355,7 @@ __device__ __forceinline__ void bp_tile_loop(const TileArgs &a, BpWord<MODE> *ac
                                              const int *count, int tx0, int tx1, int ty0,
                                              int ty1, int b, const int (&ex)[SB],
                                              const bool (&fx)[SB]) {
+  static_assert(SB <= kWeightPad, "a slice batch of weights past the last row must be allocated");
   constexpr int PITCH = TilePitch<TW, TH>::kPitch;
   constexpr int PLANE = TilePitch<TW, TH>::kPlane;
   const int N = a.g.N;
@@ -400,26 +401,42 @@ __device__ __forceinline__ void bp_tile_loop(const TileArgs &a, BpWord<MODE> *ac
     RayInfo ray;
     float w[SB];
   };
+  // Every lane issues the same loads on every path (row 0 when it has no
+  // pair; a whole slice batch of weights, which the kWeightPad floats after
+  // the weights keep in bounds for the last row): with one path, the
+  // compiler's wait for the current pair's data covers that pair only, and
+  // not the next pair's loads issued after it.
   auto fetch = [&](int r, int p, int k, Pair &q) {
-    q.ok = false;
-    if (!live(r, p)) return;
+    const bool lv = live(r, p);
+    const int pp = lv ? p : 0;
+    bool ok = lv;
     if (kRank) {
-      const int cnt = count[p] + extra;
-      if (r >= cnt) return;
+      const int cnt = count[pp] + extra;
+      ok = ok && r < cnt;
       k = ((cnt - 1) >> 1) + ((r & 1) ? (r + 1) >> 1 : -(r >> 1));
     }
     int cmin, cmax;
-    const AngleInfo ang = a.ang[p];
+    const AngleInfo ang = a.ang[pp];
     tile_columns(ang, x0, x1, y0, y1, a.yi0, cmin, cmax);
     const int c = cmin + k;
-    if (c < 0 || c >= C || c > cmax) return;
-    q.ok = true;
+    ok = ok && c >= 0 && c < C && c <= cmax;
+    q.ok = ok;
     q.quadrant = ang.quadrant;
-    q.ray = a.rays[(int64_t)p * C + c];
+    const int64_t t = ok ? (int64_t)pp * C + c : 0;
+    q.ray = a.rays[t];
     if (MODE == kBpSino) {
-      const float *wr = a.w + ((int64_t)p * C + c) * a.S + b * SB;
+      const float *wr = a.w + t * a.S + b * SB;
+      if constexpr (SB % 4 == 0) {
 #pragma unroll
-      for (int i = 0; i < SB; ++i) q.w[i] = (b * SB + i < a.S) ? wr[i] : 0.f;
+        for (int i = 0; i < SB; i += 4) {
+          float4 v;
+          __builtin_memcpy(&v, wr + i, sizeof v);  // rows are 4-byte aligned
+          q.w[i] = v.x, q.w[i + 1] = v.y, q.w[i + 2] = v.z, q.w[i + 3] = v.w;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < SB; ++i) q.w[i] = wr[i];
+      }
     }
   };
   int r = 0, p = 0, k = threadIdx.x;
@@ -445,8 +462,10 @@ __device__ __forceinline__ void bp_tile_loop(const TileArgs &a, BpWord<MODE> *ac
         any = false;
 #pragma unroll
         for (int q = 0; q < SB; ++q) {
-          any |= (cur.w[q] != 0.f);
-          wv[q] = (ALLFX || fx[q]) ? ldexpf(cur.w[q], ex[q]) : cur.w[q];
+          // slots past the stack's last slice (read from the next row) add 0
+          const float wq = b * SB + q < a.S ? cur.w[q] : 0.f;
+          any |= (wq != 0.f);
+          wv[q] = (ALLFX || fx[q]) ? ldexpf(wq, ex[q]) : wq;
         }
       } else if (MODE == kBpOnes) {
         wv[0] = ldexpf(1.0f, ex[0]);

@@ -209,6 +209,9 @@ constexpr int kOSlots = 3;   // candidate columns per pixel and projection
 constexpr int kODups = 2;    // segments per ray and pixel
 constexpr int kOWave = 64;
 
+// projection slot (0 .. kOA-1) of a lane's phase-1 pair
+__device__ __forceinline__ int ai1_of(int lane) { return lane / kORanks; }
+
 template <int SB>
 struct OrdTileLds {
   static constexpr int kWPitch = SB >= 4 ? SB + 4 : SB;  // row pitch of s_w (bank spread)
@@ -217,11 +220,15 @@ struct OrdTileLds {
 template <int SB, int MODE, bool kExact>
 __global__ void __launch_bounds__(kOWave)
 k_bp_ordered_tile(OrdArgs a, unsigned *__restrict__ overflow) {
+  static_assert(SB <= kWeightPad, "a slice batch of weights past the last row must be allocated");
   constexpr int WP = OrdTileLds<SB>::kWPitch;
   __shared__ unsigned s_cnt[kOA][kOWave];                  // byte counter per column slot
   __shared__ float s_len[kOA][kOSlots * kODups][kOWave];
   __shared__ float s_w[MODE == kOrdDen ? 1 : kOA * kORanks * WP];
   __shared__ int s_cmin[kOA];
+  // first candidate column of each pixel for the round's projections,
+  // written by the pixel's lane one round ahead (double-buffered)
+  __shared__ int s_pc0[2][kOA][kOWave];
   const int N = a.g.N, C = a.g.C;
   const int tiles_x = (N + kOT - 1) / kOT;
   const int tile = (int)xcd_chunked(blockIdx.x, gridDim.x);
@@ -245,12 +252,38 @@ k_bp_ordered_tile(OrdArgs a, unsigned *__restrict__ overflow) {
 #pragma unroll
     for (int q = 0; q < SB; ++q) xv[q] = src[q];
   }
+  // Wait for the pixel's values here, once.  Otherwise the compiler waits
+  // for them where phase 2 first reads them, inside the loop, and its
+  // in-order counter then also waits for the next round's prefetch issued
+  // just before, which serialises every round on that fetch.
+  wait_vector_loads();
   const float x0 = grid_line(N, tx0), x1 = grid_line(N, tx1);
   const float y0 = grid_line(N, ty0), y1 = grid_line(N, ty1);
+  // The angle records of the next round are loaded at the start of a round
+  // (nxt: the four projections, for the pixels' columns; fang: this lane's
+  // phase-1 projection, for its pair), so that they have arrived by the
+  // time they are used and their wait never covers the pair prefetch.
+  AngleInfo nxt[kOA], fang;
+  auto load_angles = [&](int p) {
+#pragma unroll
+    for (int ai = 0; ai < kOA; ++ai) nxt[ai] = a.ang[min(p + ai, a.P - 1)];
+    fang = a.ang[min(p + ai1_of(lane), a.P - 1)];
+  };
+  // this lane's pixel's first candidate column for projections p .. p+kOA-1
+  auto pixel_columns = [&](int p, int buf) {
+#pragma unroll
+    for (int ai = 0; ai < kOA; ++ai) {
+      int pc0 = 0, pc1;
+      if (p + ai < a.P) tile_columns(nxt[ai], pcx0, pcx1, pcy0, pcy1, a.yi0, pc0, pc1);
+      s_pc0[buf][ai][lane] = pc0;
+    }
+  };
 #pragma unroll
   for (int ai = 0; ai < kOA; ++ai) s_cnt[ai][lane] = 0;
+  load_angles(0);
+  pixel_columns(0, 0);
   __syncthreads();
-  const int ai1 = lane / kORanks, k1 = lane % kORanks;  // phase-1 pair of this lane
+  const int ai1 = ai1_of(lane), k1 = lane % kORanks;  // phase-1 pair of this lane
   // The lane's phase-1 pair of a round, fetched one round ahead: its ray
   // table entry and weights mostly miss L2, so the next round's loads are
   // issued before this round's phase 2 and arrive while it runs.
@@ -260,29 +293,39 @@ k_bp_ordered_tile(OrdArgs a, unsigned *__restrict__ overflow) {
     RayInfo ray;
     float w[SB];
   };
-  auto fetch = [&](int p, Pair &q) {
-    q.ok = false;
-    q.cmin = 0;
-    if (p >= a.P) return;
-    const AngleInfo ang = a.ang[p];
+  // Every lane issues the same loads on every path (row 0 when it has no
+  // pair; a whole slice batch of weights, which the kWeightPad floats after
+  // the weights keep in bounds for the last row): with one path, the
+  // compiler's wait for a load covers that load and the older ones only, so
+  // the prefetch is never waited for before its use.
+  auto fetch = [&](int p, const AngleInfo &ang, Pair &q) {
     int cmin, cmax;
     tile_columns(ang, x0, x1, y0, y1, a.yi0, cmin, cmax);
     q.cmin = cmin;
     q.quadrant = ang.quadrant;
     q.c = cmin + k1;
-    if (q.c > cmax || q.c < 0 || q.c >= C) return;
-    q.ok = true;
-    const int64_t t = (int64_t)p * C + q.c;
+    q.ok = p < a.P && q.c <= cmax && q.c >= 0 && q.c < C;
+    const int64_t t = q.ok ? (int64_t)p * C + q.c : 0;
     q.ray = a.rays[t];
     if (MODE != kOrdDen) {
       const float *wr = a.w + t * a.S + b * SB;
+      if constexpr (SB % 4 == 0) {
 #pragma unroll
-      for (int i = 0; i < SB; ++i) q.w[i] = (b * SB + i < a.S) ? wr[i] : 0.f;
+        for (int i = 0; i < SB; i += 4) {
+          float4 v;
+          __builtin_memcpy(&v, wr + i, sizeof v);  // rows are 4-byte aligned
+          q.w[i] = v.x, q.w[i + 1] = v.y, q.w[i + 2] = v.z, q.w[i + 3] = v.w;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < SB; ++i) q.w[i] = wr[i];
+      }
     }
   };
   Pair cur;
-  fetch(ai1, cur);
-  for (int p0 = 0; p0 < a.P; p0 += kOA) {
+  fetch(ai1, fang, cur);
+  for (int p0 = 0, buf = 0; p0 < a.P; p0 += kOA, buf ^= 1) {
+    load_angles(p0 + kOA);
     // phase 1: walk the pairs, file each segment of the tile in its slot
     const int p = p0 + ai1;
     if (p < a.P) {
@@ -290,24 +333,21 @@ k_bp_ordered_tile(OrdArgs a, unsigned *__restrict__ overflow) {
       if (cur.ok) {
         const int c = cur.c;
         if (MODE != kOrdDen) {
+          // slots past the stack's last slice (read from the next row) add 0
           float *dst = s_w + (ai1 * kORanks + k1) * WP;
 #pragma unroll
-          for (int q = 0; q < SB; ++q) dst[q] = cur.w[q];
+          for (int q = 0; q < SB; ++q) dst[q] = b * SB + q < a.S ? cur.w[q] : 0.f;
         }
-        const AngleInfo ang = a.ang[p];
         traverse_window<kExact, true>(a.g, cur.ray, cur.quadrant, tx0, tx1, ty0, ty1,
                                       [&](int ix, int iy, float l, float) {
           const unsigned lx = (unsigned)(ix - tx0), ly = (unsigned)(iy - ty0);
           if (lx < (unsigned)(tx1 - tx0) && ly < (unsigned)(ty1 - ty0)) {
-            int pc0, pc1;
-            tile_columns(ang, grid_line(N, ix), grid_line(N, ix + 1), grid_line(N, iy),
-                         grid_line(N, iy + 1), a.yi0, pc0, pc1);
-            const int slot = c - pc0;
+            const int lp = (int)(ly * kOT + lx);
+            const int slot = c - s_pc0[buf][ai1][lp];
             if ((unsigned)slot >= (unsigned)kOSlots) {
               atomicOr(overflow, 1u);
               return;
             }
-            const int lp = (int)(ly * kOT + lx);
             const unsigned old = atomicAdd(&s_cnt[ai1][lp], 1u << (8 * slot));
             const int dup = (int)((old >> (8 * slot)) & 0xffu);
             if (dup >= kODups) {
@@ -320,39 +360,76 @@ k_bp_ordered_tile(OrdArgs a, unsigned *__restrict__ overflow) {
       }
     }
     __syncthreads();
-    fetch(p0 + kOA + ai1, cur);  // next round's pair, in flight during phase 2
-    // phase 2: each lane adds its pixel's segments in order
-    if (pix_ok) {
+    fetch(p0 + kOA + ai1, fang, cur);  // next round's pair, in flight during phase 2
+    // phase 2: each lane adds its pixel's segments in order, (projection,
+    // column slot, segment) ascending.  With many slices per segment the
+    // lane's segments of the round are walked by one flat loop, so that the
+    // wavefront runs as many iterations as its busiest pixel has segments
+    // rather than one per slot that any lane uses; with few slices the
+    // decoding would cost more than it saves, and the loops nest.
+    auto add = [&](int ai, int kbase, int slot, int d) {
+      const float l = s_len[ai][slot * kODups + d][lane];
+      if (MODE == kOrdDen) {
+        acc[0] = acc[0] + l;
+      } else {
+        const float *wr = s_w + (ai * kORanks + kbase + slot) * WP;
+#pragma unroll
+        for (int q = 0; q < SB; ++q) {
+          if (MODE == kOrdEM)
+            acc[q] = acc[q] + l * wr[q];
+          else
+            acc[q] = acc[q] - (xv[q] * l) * wr[q];
+        }
+      }
+    };
+    if (pix_ok && SB >= 8) {
+      static_assert(kOSlots == 3 && kODups <= 3, "two-bit counts of three slots");
+      unsigned cnt[kOA];  // n0 | n1 << 2 | n2 << 4 per projection
+      int kb[kOA], end[kOA];
+      int total = 0;
 #pragma unroll
       for (int ai = 0; ai < kOA; ++ai) {
         const unsigned word = s_cnt[ai][lane];
-        if (word == 0 || p0 + ai >= a.P) continue;
-        const AngleInfo ang = a.ang[p0 + ai];
-        int pc0, pc1;
-        tile_columns(ang, pcx0, pcx1, pcy0, pcy1, a.yi0, pc0, pc1);
-        const int kbase = pc0 - s_cmin[ai];
+        s_cnt[ai][lane] = 0;
+        const unsigned n0 = min(word & 0xffu, (unsigned)kODups);
+        const unsigned n1 = min((word >> 8) & 0xffu, (unsigned)kODups);
+        const unsigned n2 = min((word >> 16) & 0xffu, (unsigned)kODups);
+        cnt[ai] = n0 | (n1 << 2) | (n2 << 4);
+        total += (int)(n0 + n1 + n2);
+        end[ai] = total;
+        kb[ai] = s_pc0[buf][ai][lane] - s_cmin[ai];
+      }
+      for (int k = 0; k < total; ++k) {
+        // projection of the k-th segment, and its rank r within it
+        int ai = 0, base = 0;
+#pragma unroll
+        for (int i = 0; i < kOA - 1; ++i)
+          if (k >= end[i]) ai = i + 1, base = end[i];
+        unsigned c = cnt[0];
+        int kbase = kb[0];
+#pragma unroll
+        for (int i = 1; i < kOA; ++i)
+          if (ai == i) c = cnt[i], kbase = kb[i];
+        const int r = k - base;
+        const int n0 = (int)(c & 3u), n01 = n0 + (int)((c >> 2) & 3u);
+        const int slot = (r >= n0) + (r >= n01);
+        add(ai, kbase, slot, r - (slot == 0 ? 0 : (slot == 1 ? n0 : n01)));
+      }
+    } else if (pix_ok) {
+#pragma unroll
+      for (int ai = 0; ai < kOA; ++ai) {
+        const unsigned word = s_cnt[ai][lane];
+        if (word == 0) continue;
+        s_cnt[ai][lane] = 0;
+        const int kbase = s_pc0[buf][ai][lane] - s_cmin[ai];
 #pragma unroll
         for (int slot = 0; slot < kOSlots; ++slot) {
           const int n = min((int)((word >> (8 * slot)) & 0xffu), kODups);
-          for (int d = 0; d < n; ++d) {
-            const float l = s_len[ai][slot * kODups + d][lane];
-            if (MODE == kOrdDen) {
-              acc[0] = acc[0] + l;
-            } else {
-              const float *wr = s_w + (ai * kORanks + kbase + slot) * WP;
-#pragma unroll
-              for (int q = 0; q < SB; ++q) {
-                if (MODE == kOrdEM)
-                  acc[q] = acc[q] + l * wr[q];
-                else
-                  acc[q] = acc[q] - (xv[q] * l) * wr[q];
-              }
-            }
-          }
+          for (int d = 0; d < n; ++d) add(ai, kbase, slot, d);
         }
-        s_cnt[ai][lane] = 0;
       }
     }
+    pixel_columns(p0 + kOA, buf ^ 1);
     __syncthreads();
   }
   if (!pix_ok) return;
