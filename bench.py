@@ -14,12 +14,16 @@ per second (= iterations per second per 2048^2 slice, summed over slices).
 `roofline` reports the dominant kernel of the step against the HBM roofline
 (8 TB/s).  The ray kernels regenerate the system matrix on the fly, so no
 byte model of a stored operator bounds them; what they move is measured:
-  achieved  = HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
-              calibrated on a copy kernel; profiles/traffic.json) / the
-              kernel's average launch time, timed live with HIP events on
-              the launch stream;
-  frac      = achieved / 8 TB/s (<= 1 by construction);
-  traffic   = those HBM bytes per launch;
+  achieved  = memory-side bytes per launch (rocprofv3's size-resolved
+              L2-to-memory request counters, TCC_EA0_RDREQ_32B/64B/128B and
+              TCC_EA0_WRREQ/_64B, exact on a copy kernel and in agreement
+              with FETCH_SIZE and WRITE_SIZE times the copy's factors;
+              profiles/traffic.json) / the kernel's average launch time,
+              timed live with HIP events on the launch stream;
+  frac      = achieved / 8 TB/s;
+  traffic   = those bytes per launch, an upper bound on HBM bytes (the
+              Infinity Cache serves part of them);
+  copy_GBs  = what a 1 GiB device copy reaches on the same GPU;
   compulsory_bytes_per_launch = what the kernel cannot avoid moving: its
               image, sinogram and weight arrays once, and the ray table once
               per slice batch; traffic_over_compulsory is the re-read factor;
@@ -171,7 +175,7 @@ def _code_only(text: str) -> str:
 # The device code: the kernels and what they include.  Host files (the
 # C-ABI, the batch set-up) choose launch shapes, which every counter record
 # names separately.
-KERNEL_SOURCES = ("internal.h", "kernels.hip", "traverse.h")
+KERNEL_SOURCES = ("internal.h", "kernels.hip", "ordered.hip", "traverse.h")
 
 
 def kernel_source_hash(d: str | None = None) -> str:
@@ -187,11 +191,21 @@ def kernel_source_hash(d: str | None = None) -> str:
     return h.hexdigest()[:16]
 
 
+def traffic_calibration() -> dict:
+    """The copy-kernel calibration of the traffic counters (profiles/
+    traffic.json), or {}."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            return json.load(f).get("calibration", {})
+    except Exception:
+        return {}
+
+
 def traffic_record(kernel, workload, shape=None):
-    """Counter-derived HBM bytes per launch (FETCH_SIZE and WRITE_SIZE
-    scaled by a copy-kernel calibration) committed under profiles/, or None
-    unless a record matches this kernel, workload, launch shape and kernel
-    sources."""
+    """Counter-derived memory-side bytes per launch (size-resolved L2
+    request counters, checked on a copy kernel) committed under profiles/,
+    or None unless a record matches this kernel, workload, launch shape and
+    kernel sources."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(p) as f:
@@ -434,7 +448,8 @@ def run_stack(args):
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
             "traffic": traffic,
-            "achieved_from": "counter HBM bytes per launch / HIP-event launch time",
+            "achieved_from": "counter memory-side bytes per launch / HIP-event launch time",
+            "copy_GBs": traffic_calibration().get("copy_1GiB_GBs"),
             "avg_launch_ms": t_dom * 1e3,
             "compulsory_bytes_per_launch": kd["compulsory_bytes_per_launch"],
             "traffic_over_compulsory": kd["traffic_over_compulsory"],
@@ -566,6 +581,7 @@ def run_regularised(args):
                      "avg_launch_ms": t_upd * 1e3},
         "slice_iter_per_s": slice_iters / elapsed,
         "phases_ms_per_iter": {k: v[0] / max(v[1], 1) for k, v in phases.items()},
+        "back_kernel": back_kernel_record(phases, workload),
         "stored_operator_frac": (S * (16.0 * nnz + 12.0 * R + 24.0 * N * N))
                                 / (elapsed / (iters * args.steps)) / 1e9 / HBM_PEAK_GBS,
         "launch_shape": {"fwd_batch_tile_w_h_batch": list(pj.launch_shape()),
@@ -576,6 +592,18 @@ def run_regularised(args):
                    "nonnegative": bool((x >= 0).all().item())},
     }
     print(json.dumps(out), flush=True)
+
+
+def back_kernel_record(phases, workload):
+    """The backprojection of a PML/APMLR iteration (the per-pixel sums in
+    the reference's order, k_bp_ordered_tile): launch time and counter
+    traffic."""
+    ms, n = phases["back"]
+    t = ms / max(n, 1) / 1e3
+    traffic = traffic_record("k_bp_ordered_tile", workload, [32])
+    return {"kernel": "k_bp_ordered_tile", "avg_launch_ms": t * 1e3,
+            "traffic_bytes_per_launch": traffic,
+            "memory_GBs": traffic / t / 1e9 if (traffic and t > 0) else None}
 
 
 def run_autograd(args):

@@ -55,3 +55,23 @@ def test_committed_records_match_the_current_kernels():
     assert recs
     h = bench.kernel_source_hash()
     assert all(r["source_sha"] == h for r in recs)
+
+
+def test_counter_records_match_the_current_kernels():
+    recs = json.load(open(os.path.join(ROOT, "profiles", "counters.json")))["records"]
+    assert {r["kernel"] for r in recs} >= {"k_forward", "k_bp_tile", "k_update_em"}
+    h = bench.kernel_source_hash()
+    assert all(r["source_sha"] == h and r["limiter"] for r in recs)
+
+
+def test_traffic_records_are_consistent():
+    """The size-resolved request counts reproduce the copy kernel's bytes,
+    and every record's bytes agree with FETCH_SIZE and WRITE_SIZE scaled by
+    the copy's factors (all of these kernels read whole 128-byte lines)."""
+    t = json.load(open(os.path.join(ROOT, "profiles", "traffic.json")))
+    cal = t["calibration"]
+    assert abs(cal["request_sized_read_error"]) < 1e-3
+    assert abs(cal["request_sized_write_error"]) < 1e-3
+    for r in t["records"]:
+        assert r["bytes_per_launch"] == r["read_bytes"] + r["write_bytes"]
+        assert abs(r["bytes_per_launch"] / r["fetch_write_calibrated_bytes"] - 1) < 0.02, r["kernel"]
