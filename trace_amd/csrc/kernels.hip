@@ -401,22 +401,33 @@ __device__ __forceinline__ void bp_tile_loop(const TileArgs &a, BpWord<MODE> *ac
     RayInfo ray;
     float w[SB];
   };
+  // A pair's slot resolved to (angle, column slot), and whether it exists
+  // so far (the column range is checked when the pair is fetched).
+  struct Slot {
+    int p, k;
+    bool ok;
+  };
+  auto resolve = [&](int r, int p, int k) {
+    const bool lv = live(r, p);
+    Slot o{lv ? p : 0, k, lv};
+    if (kRank) {
+      const int cnt = count[o.p] + extra;
+      o.ok = o.ok && r < cnt;
+      o.k = ((cnt - 1) >> 1) + ((r & 1) ? (r + 1) >> 1 : -(r >> 1));
+    }
+    return o;
+  };
   // Every lane issues the same loads on every path (row 0 when it has no
   // pair; a whole slice batch of weights, which the kWeightPad floats after
   // the weights keep in bounds for the last row): with one path, the
   // compiler's wait for the current pair's data covers that pair only, and
-  // not the next pair's loads issued after it.
-  auto fetch = [&](int r, int p, int k, Pair &q) {
-    const bool lv = live(r, p);
-    const int pp = lv ? p : 0;
-    bool ok = lv;
-    if (kRank) {
-      const int cnt = count[pp] + extra;
-      ok = ok && r < cnt;
-      k = ((cnt - 1) >> 1) + ((r & 1) ? (r + 1) >> 1 : -(r >> 1));
-    }
+  // not the next pair's loads issued after it.  The angle record comes in
+  // loaded a pair earlier still, so no wait at all sits between a pair's
+  // resolution and its loads.
+  auto fetch = [&](const Slot &sl, const AngleInfo &ang, Pair &q) {
+    const int pp = sl.p, k = sl.k;
+    bool ok = sl.ok;
     int cmin, cmax;
-    const AngleInfo ang = a.ang[pp];
     tile_columns(ang, x0, x1, y0, y1, a.yi0, cmin, cmax);
     const int c = cmin + k;
     ok = ok && c >= 0 && c < C && c <= cmax;
@@ -447,12 +458,22 @@ __device__ __forceinline__ void bp_tile_loop(const TileArgs &a, BpWord<MODE> *ac
     while (p < a.P && k >= count[p] + extra) { k -= count[p] + extra; ++p; }
   }
   Pair cur;
-  fetch(r, p, k, cur);
+  {
+    const Slot s0 = resolve(r, p, k);
+    fetch(s0, a.ang[s0.p], cur);
+  }
+  int rn = r, pn = p, kn = k;
+  advance(rn, pn, kn);
+  Slot sn = resolve(rn, pn, kn);
+  AngleInfo angn = a.ang[sn.p];
   while (live(r, p)) {
-    int rn = r, pn = p, kn = k;
-    advance(rn, pn, kn);
+    // the pair after next: slot and angle record
+    int rnn = rn, pnn = pn, knn = kn;
+    advance(rnn, pnn, knn);
+    const Slot snn = resolve(rnn, pnn, knn);
+    const AngleInfo angnn = a.ang[snn.p];
     Pair nxt;
-    fetch(rn, pn, kn, nxt);
+    fetch(sn, angn, nxt);
     if (cur.ok) {
       // weights pre-scaled by 2^e (exact: a power-of-two scaling commutes
       // with rounding), so fl(len * ws) = fl(len * w) * 2^e per segment
@@ -532,6 +553,11 @@ __device__ __forceinline__ void bp_tile_loop(const TileArgs &a, BpWord<MODE> *ac
     p = pn;
     k = kn;
     cur = nxt;
+    rn = rnn;
+    pn = pnn;
+    kn = knn;
+    sn = snn;
+    angn = angnn;
   }
 }
 
