@@ -388,6 +388,7 @@ def run_stack(args):
             "hbm_GBs": traffic / t / 1e9 if (traffic and t > 0) else None,
             "compulsory_bytes_per_launch": comp,
             "traffic_over_compulsory": traffic / comp if traffic else None,
+            "stored_operator_bytes_per_launch": stored,
             "stored_operator_equiv_GBs": stored / t / 1e9 if t > 0 else None,
         }
     dom = max(("forward", "back"), key=lambda k: per_kernel[k]["avg_launch_ms"])
@@ -454,6 +455,10 @@ def run_stack(args):
             "compulsory_bytes_per_launch": kd["compulsory_bytes_per_launch"],
             "traffic_over_compulsory": kd["traffic_over_compulsory"],
             "limiter": limiter,
+            "stored_operator_bytes_per_launch": kd["stored_operator_bytes_per_launch"],
+            "stored_operator_model": ("per slice 8*nnz + 8*R + 4*N^2 (forward) / 8*nnz + 4*R "
+                                      "+ 4*N^2 (back): an int32 index and an fp32 length per "
+                                      "nonzero of a stored A, for comparison only"),
             "stored_operator_equiv_GBs": kd["stored_operator_equiv_GBs"],
             "stored_operator_step_equiv_GBs": b_stored_step / (elapsed / args.steps) / 1e9,
             "nnz_per_slice": nnz,
