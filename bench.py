@@ -414,7 +414,8 @@ def run_stack(args):
 
     batched = None
     if not args.no_batched:
-        batched = batched_compare(args.batch_slices, args.batched_iters)
+        batched = batched_compare(args.batch_slices, args.batched_iters,
+                                  autotune=not args.no_autotune)
 
     value = S * args.steps / elapsed
     out = {
@@ -732,18 +733,21 @@ def run_autograd(args):
     }
     print(json.dumps(out), flush=True)
 
-def batched_compare(n, iters):
+def batched_compare(n, iters, autotune=True):
     """Uneven slices (widths {256..2048}, 180-1800 angles, SIRT or MLEM,
     `iters` iterations) reconstructed (a) slice by slice with the plain
     per-slice entry point, each slice with the shape the autotuner picks for
     its width, and (b) all together by one graph launch of the batched
-    persistent kernels with the shape the batch autotuner picks.  Setup
-    (tables, graph instantiation) is outside both timings; every slice's two
-    results are compared bit for bit.  Returns a dict."""
+    persistent kernels with the shape the batch autotuner picks.  Without
+    `autotune` both keep the library's default shapes (and a kernel trace
+    of the run holds only the timed work, one warm-up iteration per plain
+    slice and the set-up).  Setup (tables, graph instantiation) is outside
+    both timings; every slice's two results are compared bit for bit.
+    Returns a dict."""
     import torch
 
     from trace_amd import Projector
-    from trace_amd.autotune import autotune
+    from trace_amd.autotune import autotune as autotune_shape
     from trace_amd.batch import Batch, autotune_batch, uneven_slices, uneven_specs
 
     dev = torch.device("cuda:0")
@@ -766,8 +770,12 @@ def batched_compare(n, iters):
         i = next(k for k, sp in enumerate(specs) if sp[0] == C)
         s = slices[i]
         with Projector(s.theta, 1, C, s.center) as tp:
-            x = s.recon.clone().reshape(1, C, C)
-            shapes[C] = autotune(tp, s.sino.reshape(-1, 1, C).contiguous(), x, s.alg)
+            if autotune:
+                x = s.recon.clone().reshape(1, C, C)
+                shapes[C] = autotune_shape(tp, s.sino.reshape(-1, 1, C).contiguous(), x, s.alg)
+            else:
+                shapes[C] = {"shape": list(tp.launch_shape()), "bundle": tp.fwd_bundle(),
+                             "lanes": tp.fwd_lanes(), "forced": True}
     handles = []
     for s in slices:
         C = s.recon.shape[0]
@@ -778,7 +786,8 @@ def batched_compare(n, iters):
         nnz_total += pj.count_segments()[0]
         handles.append(pj)
     torch.cuda.synchronize()
-    log(f"plain set-up (per-width autotune + {n} handles) in {time.perf_counter() - t0:.1f} s")
+    log(f"plain set-up ({'per-width autotune + ' if autotune else ''}{n} handles) in "
+        f"{time.perf_counter() - t0:.1f} s")
     sinos = [s.sino.reshape(-1, 1, s.recon.shape[0]).contiguous() for s in slices]
     outs = [x.clone().reshape(1, *x.shape) for x in init]
     torch.cuda.synchronize()
@@ -794,15 +803,21 @@ def batched_compare(n, iters):
 
     # (b) batched
     t0 = time.perf_counter()
-    tune = autotune_batch(slices, bundles=(2, 4, 8, 16), tiles=((32, 32), (64, 64)),
-                          sample=min(24, n), iters=1)
+    if autotune:
+        tune = autotune_batch(slices, bundles=(2, 4, 8, 16), tiles=((32, 32), (64, 64)),
+                              sample=min(24, n), iters=1)
     for s, x0 in zip(slices, init):
         s.recon.copy_(x0)
     b = Batch(slices)
-    b.set_launch_shape(tune["bundle"], *tune["tile"])
+    if autotune:
+        b.set_launch_shape(tune["bundle"], *tune["tile"])
+    else:
+        bundle, tw, th = b.launch_shape()
+        tune = {"bundle": bundle, "tile": (tw, th), "ms": None}
     b.prepare()
     torch.cuda.synchronize()
-    log(f"batch set-up (autotune {tune['bundle']}/{tune['tile']}, tables, graph) in "
+    log(f"batch set-up ({'autotune ' if autotune else 'shape '}{tune['bundle']}/{tune['tile']}, "
+        f"tables, graph) in "
         f"{time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     b.run()
@@ -840,7 +855,7 @@ def batched_compare(n, iters):
 def run_batched(args):
     """Config 3 on its own line: 200 uneven slices, 25 iterations each
     (batched_compare)."""
-    r = batched_compare(args.batch_slices, args.iters or 25)
+    r = batched_compare(args.batch_slices, args.iters or 25, autotune=not args.no_autotune)
     out = {
         "metric": "batched uneven slices/s (one graph launch) vs plain per-slice launches",
         "value": r["slices"] / r["batched_s"],

@@ -51,7 +51,8 @@ struct trace_batch {
   unsigned *counters = nullptr;     // [3]
   unsigned *maxw = nullptr;         // [n]
   int *flag = nullptr;
-  int tw = 32, th = 32, bundle = 8;
+  // the batch autotuner's usual pick on uneven slices of 256-2048 pixels
+  int tw = 64, th = 64, bundle = 8;
   bool exact = true;
   bool prepared = false;
   int grid[3] = {0, 0, 0};

@@ -126,7 +126,13 @@ int backproject(trace_handle *h, int mode, const float *w, float *z, int phase) 
 int ensure_den(trace_handle *h) {
   const bool ord = h->ordered >= 1;
   if (h->den_ready && h->den_exact == h->exact && h->den_ordered == ord) return TRACE_OK;
-  if (int rc = backproject(h, kBpOutOnes, nullptr, h->den, TRACE_PHASE_SETUP)) return rc;
+  // the den the updates divide by: the reference's float sum in its order,
+  // or the exact fixed-point sum rounded once
+  if (ord) {
+    if (int rc = ordered_den(h, h->den)) return rc;
+  } else {
+    if (int rc = backproject(h, kBpOutOnes, nullptr, h->den, TRACE_PHASE_SETUP)) return rc;
+  }
   // largest column sum: bounds every pixel's sum of segment lengths, which
   // sets the 32-bit fixed-point scale of the weights backprojector
   unsigned *bits = reinterpret_cast<unsigned *>(h->flag);
@@ -138,12 +144,12 @@ int ensure_den(trace_handle *h) {
   HIP_TRY(hipStreamSynchronize(h->stream));
   float m = 0.f;
   std::memcpy(&m, &b, sizeof(m));
-  // den itself carries a fixed-point error far below 2^-20 relative
-  h->den_max = std::isfinite(m) && m > 0.f ? m * (1.0f + 0x1p-16f) : 1.0f;
-  // the den the updates divide by: the reference's float sum in its order
-  // (the fixed-point sum above is exact to 2^-20 but rounds differently)
-  if (ord)
-    if (int rc = ordered_den(h, h->den)) return rc;
+  // The fixed-point den is exact to far below 2^-20 relative.  The float
+  // sum in order is within (n - 1) * 2^-24 of the exact one for n terms,
+  // and a pixel takes at most 6 segments per projection (3 candidate
+  // columns, 2 segments each), so the margin covers 6 P terms.
+  const double margin = 1.0 + 0x1p-16 + (ord ? 6.0 * h->geo.P * 0x1p-24 : 0.0);
+  h->den_max = std::isfinite(m) && m > 0.f ? (float)(m * margin) : 1.0f;
   h->den_ready = true;
   h->den_exact = h->exact;
   h->den_ordered = ord;
@@ -151,9 +157,8 @@ int ensure_den(trace_handle *h) {
 }
 
 // Slices per thread of the ordered backprojector: up to 32 in the tile form
-// (16 for kOrdNum, whose per-pixel image values also take registers) and 8
-// in the per-pixel form, a divisor of the slice-batched copy's batch for
-// kOrdNum.
+// and 8 in the per-pixel form, a divisor of the slice-batched copy's batch
+// for kOrdNum.
 static int ordered_batch(int mode, int S, int xsb, bool tile) {
   int sb = tile ? 32 : 8;
   while (sb > 1 && sb / 2 >= S) sb /= 2;
@@ -410,8 +415,15 @@ int trace_create(trace_handle_t **out, const trace_geom_t *g, int device,
     h->fwd_sb = 8, h->fwd_lanes = 2, h->tile = TileShape{16, 16, 8};
   } else if (S >= 3) {
     h->fwd_sb = 4, h->fwd_lanes = 1, h->tile = TileShape{32, 32, 4};
+  } else if (S == 2) {
+    h->fwd_sb = 2, h->fwd_lanes = 1, h->tile = TileShape{32, 32, 2};
   } else {
-    h->fwd_sb = S, h->fwd_lanes = 1, h->tile = TileShape{32, 32, S};
+    // one slice: one atomic per segment, so larger tiles pay off as the
+    // image grows (the autotuner's picks at 1500 angles: 16x16 up to 512
+    // pixels, 32x32 at 1024 and 1536, 64x64 at 2048)
+    const int n = g->num_grids > 0 ? g->num_grids : g->num_cols;
+    const int side = n >= 1792 ? 64 : (n >= 768 ? 32 : 16);
+    h->fwd_sb = 1, h->fwd_lanes = 1, h->tile = TileShape{side, side, 1};
   }
   // 16 angles x 1 column per wavefront at four lanes per ray (measured at
   // 2048^2 / 1500 angles: 110 vs 116 ms for 8 angles x 2 columns), 8 x 8/LPR
