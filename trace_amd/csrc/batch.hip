@@ -81,7 +81,7 @@ void slice_items(const trace_batch *b, int i, int64_t out[3]) {
   const int64_t bundles = (int64_t)((P + ba - 1) / ba) * ((C + bc - 1) / bc);
   out[0] = (bundles + 3) / 4;  // four wavefront bundles per 256-lane workgroup
   out[1] = (int64_t)((N + b->tw - 1) / b->tw) * ((N + b->th - 1) / b->th);
-  out[2] = ((int64_t)N * N + 255) / 256;
+  out[2] = ((int64_t)N * N + 256 * kBatchUpdateBlocks - 1) / (256 * kBatchUpdateBlocks);
 }
 
 // Tables, descriptors and launch grids for the current launch shape.

@@ -32,6 +32,9 @@ __device__ __forceinline__ void wait_vector_loads() { __builtin_amdgcn_s_waitcnt
 // row's partial batch stays inside the allocation.
 constexpr int kWeightPad = 32;
 
+// Pixel blocks (of 256 pixels) per work item of the batched image update.
+constexpr int kBatchUpdateBlocks = 16;
+
 enum Alg { kSIRT = 0, kMLEM = 1, kPML = 2, kAPMLR = 3 };
 enum FwdModeHost { kFwdOutY = 0, kFwdOutSIRT = 1, kFwdOutEM = 2 };
 enum BpModeHost { kBpOutSino = 0, kBpOutOnes = 1, kBpOutCount = 2 };

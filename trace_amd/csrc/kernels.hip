@@ -1061,8 +1061,10 @@ k_bp_batch(const BatchSliceDev *__restrict__ d, BatchQueues q, int it) {
   }
 }
 
-// SIRT (sirt.cc:21-33) / MLEM (mlem.cc:21-34) update, kPixBlock pixels per
-// item.
+// SIRT (sirt.cc:21-33) / MLEM (mlem.cc:21-34) update, kBatchUpdateBlocks
+// blocks of kPixBlock pixels per item (one item per block made the shared
+// item counter, ~1.2 million atomics per iteration at 200 uneven slices,
+// cost 14 ms where the pixels take 1.6 ms).
 __global__ void __launch_bounds__(kPixBlock)
 k_update_batch(const BatchSliceDev *__restrict__ d, BatchQueues q, int it) {
   const int *start = q.start[2];
@@ -1074,12 +1076,16 @@ k_update_batch(const BatchSliceDev *__restrict__ d, BatchQueues q, int it) {
     const BatchSliceDev &sd = d[__builtin_amdgcn_readfirstlane(q.order[2][k])];
     if (it >= sd.iters) continue;
     const int64_t NN = (int64_t)sd.g.N * sd.g.N;
-    const int64_t j = (int64_t)(item - start[k]) * kPixBlock + threadIdx.x;
-    if (j < NN) {
-      float xv = sd.x[j];
-      if (sd.alg == 0) xv += sd.z[j] / sd.den[j];
-      else xv *= sd.z[j] / sd.den[j];
-      sd.x[j] = xv;
+    const int64_t j0 = (int64_t)(item - start[k]) * kPixBlock * kBatchUpdateBlocks + threadIdx.x;
+#pragma unroll 4
+    for (int u = 0; u < kBatchUpdateBlocks; ++u) {
+      const int64_t j = j0 + (int64_t)u * kPixBlock;
+      if (j < NN) {
+        float xv = sd.x[j];
+        if (sd.alg == 0) xv += sd.z[j] / sd.den[j];
+        else xv *= sd.z[j] / sd.den[j];
+        sd.x[j] = xv;
+      }
     }
   }
 }
