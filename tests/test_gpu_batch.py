@@ -141,3 +141,21 @@ def test_batch_nonfinite(gpu):
     with Batch(sl) as b:
         with pytest.raises(TraceNonFinite):
             b.run()
+
+
+@pytest.mark.parametrize("N,side", [(256, 16), (767, 16), (768, 32), (1536, 32), (1792, 64),
+                                    (2048, 64)])
+def test_one_slice_default_tile_follows_image_side(gpu, N, side):
+    """Without autotuning a one-slice handle takes 16x16 tiles up to 767
+    pixels, 32x32 up to 1791 and 64x64 above (the autotuner's picks at 1500
+    angles); the batch defaults to 8-angle bundles and 64x64 tiles."""
+    from trace_amd import Projector
+    from trace_amd.batch import Batch
+    from trace_amd.phantom import angles_rad
+    th = angles_rad(30)
+    with Projector(th, 1, N, N / 2.0) as pj:
+        assert pj.launch_shape() == (1, side, side, 1)
+    if N == 256:
+        sl = _make(_specs()[:2], seed=3)
+        with Batch(sl) as b:
+            assert b.launch_shape() == (8, 64, 64)
