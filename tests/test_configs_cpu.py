@@ -1,6 +1,6 @@
 """The full-size configuration fixtures (tests/golden/configs.npz, .json) on
 the CPU: their layout, that the inputs rebuild to the recorded sinograms, and
-that the smallest case (config 3's 256^2 slice, 959 angles, 25 SIRT
+that the smallest case (the 256^2 uneven slice, 959 angles, 25 SIRT
 iterations) is what the oracle computes today, bit for bit."""
 import json
 import os

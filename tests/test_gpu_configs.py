@@ -3,11 +3,11 @@ to the reference at one thread), through fixtures written by
 tests/golden/make_config_fixtures.py: sampled pixels and per-row sums of the
 oracle's fp32 result and of its fp64 twin.
 
-  config 1  MLEM 1024^2, 720 angles, 50 iterations      (mlem_main.cc:171-182)
-  config 2  PML 2048^2, 1500 angles, 30 iterations      (pml_main.cc:178-193)
-            APMLR 2 x 2048^2, 1500 angles, 30 iterations (apmlr_main.cc:209-244)
-  config 3  four of the 200 uneven slices (widths 256, 1024, 1536, 2048),
-            25 iterations, through the batched graph path
+  cfg1   MLEM 1024^2, 720 angles, 50 iterations      (mlem_main.cc:171-182)
+  cfg2p  PML 2048^2, 1500 angles, 30 iterations      (pml_main.cc:178-193)
+  cfg2a  APMLR 2 x 2048^2, 1500 angles, 30 iterations (apmlr_main.cc:209-244)
+  cfg3_i four of 200 uneven slices (widths 256, 1024, 1536, 2048),
+         25 iterations, through the batched graph path
 
 Bar: relative L2 <= 1e-4 against the reference on the sampled pixels and on
 the row sums.  PML and APMLR sum den and num in the reference's order, so
@@ -66,12 +66,12 @@ def test_config_vs_reference(gpu, key):
     from trace_amd import Projector, RegParams
     cases, arr = _fixture()
     meta = cases[key]
-    print(f"config {key}: building inputs", flush=True)
+    print(f"case {key}: building inputs", flush=True)
     th, sino, init = _inputs(key, meta)
     P, S, C = sino.shape
     prm = meta["params"]
     rp = RegParams(**prm) if prm else None
-    print(f"config {key}: {meta['alg']} {S}x{C}^2, {P} angles, {meta['iters']} iterations",
+    print(f"case {key}: {meta['alg']} {S}x{C}^2, {P} angles, {meta['iters']} iterations",
           flush=True)
     with Projector(th, S, C, C / 2.0) as pj:
         ds = torch.from_numpy(sino).cuda()
@@ -90,13 +90,13 @@ def test_config3_batched_vs_reference(gpu):
     slices = []
     for key in keys:
         meta = cases[key]
-        print(f"config {key}: building inputs", flush=True)
+        print(f"case {key}: building inputs", flush=True)
         th, sino, init = _inputs(key, meta)
         C = sino.shape[2]
         slices.append(BatchSlice(th, torch.from_numpy(sino[:, 0, :].copy()).cuda(),
                                  torch.from_numpy(init[0].copy()).cuda(), meta["alg"],
                                  meta["iters"], C / 2.0))
-    print("config 3: four uneven slices through the batched graph", flush=True)
+    print("four uneven slices through the batched graph", flush=True)
     with Batch(slices) as b:
         b.run()
         torch.cuda.synchronize()
