@@ -75,3 +75,18 @@ def test_traffic_records_are_consistent():
     for r in t["records"]:
         assert r["bytes_per_launch"] == r["read_bytes"] + r["write_bytes"]
         assert abs(r["bytes_per_launch"] / r["fetch_write_calibrated_bytes"] - 1) < 0.02, r["kernel"]
+
+
+def test_more_than_one_gpu_is_refused():
+    """bench.py measures one GPU: --gpus N > 1 exits with a message (before
+    anything touches a device)."""
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+
+
+def test_host_description():
+    threads, info = bench.host_cpus()[0], bench.cpu_model()
+    assert threads >= 1 and isinstance(info, str) and info
