@@ -17,6 +17,12 @@ test can tell that it rebuilt the same input):
           (apmlr_main.cc:209-244; APMLR couples slices)
   cfg3_i  slice i of the 200 uneven slices (trace_amd.batch.uneven_specs,
           seed 3): the first of width 256, 1024, 1536 and 2048, 25 iterations
+  stack_mlem, stack_sirt
+          MLEM and SIRT, 10 iterations, at the benchmark's geometry (2048^2,
+          1500 angles) on its slice-0 phantom (seed 2), noise drawn on the CPU
+
+With --cases the named cases are recomputed and merged into the existing
+files.
 
 Per slice the fixture keeps the fp32 and fp64 values at 2048 pixels drawn
 by PCG64(20261203 + slice index) and the float64 sum of every image row (a
@@ -46,6 +52,9 @@ CASES = {
     "cfg1": ("mlem", 1024, 720, 1, 50, 1, 1e5, 20, None),
     "cfg2p": ("pml", 2048, 1500, 1, 30, 2, 1e4, 20, dict(beta=0.1)),
     "cfg2a": ("apmlr", 2048, 1500, 2, 30, 2, 1e4, 20, APMLR_PRM),
+    # the benchmark's geometry and algorithms (its slice-0 phantom)
+    "stack_mlem": ("mlem", 2048, 1500, 1, 10, 2, 1e4, 20, None),
+    "stack_sirt": ("sirt", 2048, 1500, 1, 10, 2, 1e4, 20, None),
 }
 CFG3_WIDTHS = (256, 1024, 1536, 2048)
 CFG3_ITERS = 25
@@ -144,6 +153,10 @@ def main():
             print(f"[fixtures] {key} {kind} in {dt:.0f} s", flush=True)
             res[(key, kind)] = (x, sha)
     arrays, meta = {}, {}
+    if a.cases and os.path.exists(os.path.join(HERE, "configs.json")):
+        old = np.load(os.path.join(HERE, "configs.npz"))
+        arrays = {k: old[k] for k in old.files}
+        meta = json.load(open(os.path.join(HERE, "configs.json")))["cases"]
     for k in keys:
         x32, sha = res[(k, "f32")]
         x64, _ = res[(k, "f64")]

@@ -8,6 +8,9 @@ oracle's fp32 result and of its fp64 twin.
   cfg2a  APMLR 2 x 2048^2, 1500 angles, 30 iterations (apmlr_main.cc:209-244)
   cfg3_i four of 200 uneven slices (widths 256, 1024, 1536, 2048),
          25 iterations, through the batched graph path
+  stack_mlem, stack_sirt
+         MLEM and SIRT at the benchmark's geometry (2048^2, 1500 angles),
+         10 iterations
 
 Bar: relative L2 <= 1e-4 against the reference on the sampled pixels and on
 the row sums.  PML and APMLR sum den and num in the reference's order, so
@@ -60,7 +63,7 @@ def _check(key, x, meta, arr, exact):
         assert bad == 0
 
 
-@pytest.mark.parametrize("key", ["cfg1", "cfg2p", "cfg2a"])
+@pytest.mark.parametrize("key", ["cfg1", "cfg2p", "cfg2a", "stack_mlem", "stack_sirt"])
 def test_config_vs_reference(gpu, key):
     import torch
     from trace_amd import Projector, RegParams
