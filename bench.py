@@ -175,7 +175,8 @@ def _code_only(text: str) -> str:
 # The device code: the kernels and what they include.  Host files (the
 # C-ABI, the batch set-up) choose launch shapes, which every counter record
 # names separately.
-KERNEL_SOURCES = ("internal.h", "kernels.hip", "ordered.hip", "traverse.h")
+KERNEL_SOURCES = ("internal.h", "kernels.hip", "ordered.hip", "staged.h", "staged.hip",
+                  "traverse.h")
 
 
 def kernel_source_hash(d: str | None = None) -> str:
@@ -189,6 +190,22 @@ def kernel_source_hash(d: str | None = None) -> str:
         with open(os.path.join(d, f), encoding="utf-8") as fh:
             h.update(_code_only(fh.read()).encode())
     return h.hexdigest()[:16]
+
+
+def fwd_staged(pj) -> bool:
+    """True when the projector's forward launches run the slab-staged form
+    (32-slice batches and TRACE_OPT_FWD_STAGED on)."""
+    from trace_amd import _capi
+    return pj.launch_shape()[0] == 32 and pj.get_option(_capi.TRACE_OPT_FWD_STAGED) == 1
+
+
+def fwd_shape(pj) -> list:
+    """Launch shape of the forward kernel as the profile records name it:
+    [32, angles, columns, slab] of the staged form's bundles, else [slice
+    batch, lanes per ray, bundle angles]."""
+    if fwd_staged(pj):
+        return [32, 16, 64, 4]
+    return [pj.launch_shape()[0], pj.fwd_lanes(), pj.fwd_bundle()]
 
 
 def traffic_calibration() -> dict:
@@ -367,16 +384,22 @@ def run_stack(args):
     fb, tw, th, tb = pj.launch_shape()
     lanes, bundle = pj.fwd_lanes(), pj.fwd_bundle()
     nb_f, nb_b = -(-S // fb), -(-S // tb)
+    staged = fwd_staged(pj)
+    # slab plan of the staged form (read once per slice batch): per bundle of
+    # 16 angles x 64 columns its axis and an 8-byte window per 4-pixel slab
+    nwg = -(-P // 16) * -(-N // 64)
+    plan_b = nwg * (4.0 + 8.0 * -(-N // 4)) if staged else 0.0
     # compulsory bytes per launch: each array once, the 32-byte ray table once
     # per slice batch, den once per update batch
     model = {
-        "forward": ("k_forward", S * (4.0 * N * N + 8.0 * R) + nb_f * 32.0 * R,
+        "forward": ("k_forward_staged" if staged else "k_forward",
+                    S * (4.0 * N * N + 8.0 * R) + nb_f * (32.0 * R + plan_b),
                     S * (8.0 * nnz + 8.0 * R + 4.0 * N * N)),
         "back": ("k_bp_tile", S * (4.0 * R + 4.0 * N * N) + nb_b * 32.0 * R,
                  S * (8.0 * nnz + 4.0 * R + 4.0 * N * N)),
         "update": ("k_update_em", S * 16.0 * N * N + nb_f * 4.0 * N * N, S * 16.0 * N * N),
     }
-    shapes = {"forward": [fb, lanes, bundle], "back": [tw, th, tb], "update": [fb]}
+    shapes = {"forward": fwd_shape(pj), "back": [tw, th, tb], "update": [fb]}
     per_kernel = {}
     for ph, (kname, comp, stored) in model.items():
         ms, n = phases[ph]
@@ -468,6 +491,7 @@ def run_stack(args):
         "launch_shape": {"fwd_batch_tile_w_h_batch": list(pj.launch_shape()),
                          "fwd_bundle_angles": pj.fwd_bundle(),
                          "fwd_lanes_per_ray": pj.fwd_lanes(),
+                         "fwd_staged": fwd_staged(pj),
                          "autotuned": tune is not None and not tune.get("forced")},
         "phases_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in phases.items()},
         "cpu_baseline": cpu,
@@ -593,6 +617,7 @@ def run_regularised(args):
         "launch_shape": {"fwd_batch_tile_w_h_batch": list(pj.launch_shape()),
                          "fwd_bundle_angles": pj.fwd_bundle(),
                          "fwd_lanes_per_ray": pj.fwd_lanes(),
+                         "fwd_staged": fwd_staged(pj),
                          "autotuned": tune is not None and not tune.get("forced")},
         "checks": {"finite": bool(torch.isfinite(x).all().item()),
                    "nonnegative": bool((x >= 0).all().item())},

@@ -128,6 +128,12 @@ typedef struct trace_handle trace_handle_t;
                                     segments per ray and pixel; checked once
                                     per geometry), else per pixel; 0 always
                                     per pixel */
+#define TRACE_OPT_FWD_STAGED 8 /* form of the forward projector at 32-slice
+                                  batches (same bits either way): 1 (default)
+                                  bundles of 16 angles x 64 columns sweep the
+                                  image in slabs staged in LDS; 0: every ray
+                                  reads its pixels from HBM (the form used at
+                                  other slice batches) */
 
 /* Kernel phases reported by trace_phase_times. */
 #define TRACE_PHASE_FORWARD 0 /* forward projection and per-ray weights */

@@ -59,7 +59,7 @@ def test_committed_records_match_the_current_kernels():
 
 def test_counter_records_match_the_current_kernels():
     recs = json.load(open(os.path.join(ROOT, "profiles", "counters.json")))["records"]
-    assert {r["kernel"] for r in recs} >= {"k_forward", "k_bp_tile", "k_update_em"}
+    assert {r["kernel"] for r in recs} >= {"k_forward_staged", "k_bp_tile", "k_update_em"}
     h = bench.kernel_source_hash()
     assert all(r["source_sha"] == h and r["limiter"] for r in recs)
 

@@ -30,6 +30,7 @@ TRACE_OPT_FWD_BUNDLE = 4
 TRACE_OPT_FWD_LANES = 5
 TRACE_OPT_ORDERED_SUMS = 6
 TRACE_OPT_ORDERED_TILE = 7
+TRACE_OPT_FWD_STAGED = 8
 PHASES = ("forward", "back", "update", "setup")
 
 

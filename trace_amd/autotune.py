@@ -2,7 +2,8 @@
 backprojector.
 
 The forward projector's shape is its slice batch (slices gathered per
-segment, 1 to 32) and ray-bundle width; the backprojector's is its image tile and slice batch
+segment, 1 to 32) and ray-bundle width (at 32-slice batches its slab-staged
+form, the default there, has a single shape); the backprojector's is its image tile and slice batch
 (tile_w x tile_h pixels x tile_batch slices of LDS accumulators).  The two
 kernels are independent, so each is tuned on its own: every candidate runs
 one iteration of the real workload and the phase timer (HIP events on the
@@ -13,6 +14,8 @@ from __future__ import annotations
 
 import json
 import os
+
+from . import _capi
 
 FWD_BATCHES = (1, 2, 4, 8, 16, 32)
 BUNDLES = (1, 2, 4, 8, 16)
@@ -85,6 +88,9 @@ def autotune(pj, sino, recon, alg: str = "mlem", params=None,
     fwd_ms, back_ms = {}, {}
     tile_batches = batch_candidates(sorted({t[2] for t in tiles}), pj.S)
     base_tile = next(t for t in tiles if t[2] in tile_batches)
+    staged = pj.get_option(_capi.TRACE_OPT_FWD_STAGED) == 1
+    staged_pair = next(((ln, ba) for ln in lanes_per_ray for ba in bundles
+                        if (32, ln) in FWD_SHAPES and ba * ln <= 64), None)
     pj.set_phase_timing(True)
     try:
         for fb in batch_candidates(fwd_batches, pj.S):
@@ -93,6 +99,9 @@ def autotune(pj, sino, recon, alg: str = "mlem", params=None,
                     continue
                 for ba in bundles:
                     if ba * lanes > 64:
+                        continue
+                    # the staged form (32-slice batches) has one launch shape
+                    if fb == 32 and staged and (lanes, ba) != staged_pair:
                         continue
                     apply_shape(pj, {"shape": (fb, *base_tile), "bundle": ba, "lanes": lanes})
                     work.copy_(recon)

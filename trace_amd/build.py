@@ -17,8 +17,8 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtrace_hip.so")
-SOURCES = ["kernels.hip", "ordered.hip", "capi.hip", "batch.hip"]
-HEADERS = ["traverse.h", "internal.h", "handle.h"]
+SOURCES = ["kernels.hip", "ordered.hip", "staged.hip", "capi.hip", "batch.hip"]
+HEADERS = ["traverse.h", "internal.h", "handle.h", "staged.h"]
 
 
 def hipcc() -> str:

@@ -84,6 +84,19 @@ int ensure_rays(trace_handle *h) {
   return TRACE_OK;
 }
 
+// Slab plan of the staged forward projector (once per geometry).
+int ensure_staged(trace_handle *h) {
+  if (h->staged_ready) return TRACE_OK;
+  if (int rc = ensure_rays(h)) return rc;
+  int nwg = 0, nslab = 0, ngc = 0;
+  staged_plan_size(h->geo, nwg, nslab, ngc);
+  if (!h->staged.axis) HIP_TRY(dev_alloc(&h->staged.axis, (size_t)nwg));
+  if (!h->staged.range) HIP_TRY(dev_alloc(&h->staged.range, (size_t)nwg * nslab));
+  TIMED(TRACE_PHASE_SETUP, launch_staged_plan(h->geo, h->rays, h->staged, h->stream));
+  h->staged_ready = true;
+  return TRACE_OK;
+}
+
 int ensure_count(trace_handle *h) {
   const TileShape &t = h->tile;
   if (h->count && h->count_shape.tw == t.tw && h->count_shape.th == t.th) return TRACE_OK;
@@ -310,6 +323,12 @@ int forward(trace_handle *h, int mode, const float *sino, float *out) {
   if (mode != kFwdOutY)
     TIMED(TRACE_PHASE_SETUP,
           launch_fill(reinterpret_cast<float *>(h->maxw), h->geo.S, 0.f, h->stream));
+  if (h->fwd_staged && h->fwd_sb == kStSlices) {
+    if (int rc = ensure_staged(h)) return rc;
+    TIMED(TRACE_PHASE_FORWARD, launch_forward_staged(h->geo, h->rays, h->staged, mode, h->exact,
+                                                     h->xb, sino, out, h->maxw, h->stream));
+    return TRACE_OK;
+  }
   TIMED(TRACE_PHASE_FORWARD, launch_forward(h->geo, h->rays, h->fwd_sb, h->fwd_bundle,
                                             h->fwd_lanes, mode, h->exact, h->xb, sino, out,
                                             h->maxw, h->stream));
@@ -485,7 +504,8 @@ void trace_destroy(trace_handle_t *h) {
   h->timer.release();
   for (void *p : {(void *)h->geo.d_ang, (void *)h->geo.d_yi, (void *)h->rays, (void *)h->count,
                   (void *)h->den, (void *)h->xb, (void *)h->w, (void *)h->z, (void *)h->xa,
-                  (void *)h->xc, (void *)h->flag, (void *)h->counters, (void *)h->maxw})
+                  (void *)h->xc, (void *)h->flag, (void *)h->counters, (void *)h->maxw,
+                  (void *)h->staged.axis, (void *)h->staged.range})
     dev_free(p);
   delete h;
 }
@@ -529,6 +549,10 @@ int trace_set_option(trace_handle_t *h, int option, int value) {
         return fail(TRACE_E_UNSUPPORTED, "slice batch and lanes per ray not compiled");
       h->fwd_lanes = value;
       return TRACE_OK;
+    case TRACE_OPT_FWD_STAGED:
+      if (value != 0 && value != 1) return fail(TRACE_E_INVALID, "value must be 0 or 1");
+      h->fwd_staged = value;
+      return TRACE_OK;
     default:
       return fail(TRACE_E_UNSUPPORTED, "unknown option");
   }
@@ -545,6 +569,7 @@ int trace_get_option(trace_handle_t *h, int option, int *value) {
     case TRACE_OPT_FWD_LANES: *value = h->fwd_lanes; return TRACE_OK;
     case TRACE_OPT_ORDERED_SUMS: *value = h->ordered; return TRACE_OK;
     case TRACE_OPT_ORDERED_TILE: *value = h->ord_tile != 0 ? 1 : 0; return TRACE_OK;
+    case TRACE_OPT_FWD_STAGED: *value = h->fwd_staged; return TRACE_OK;
     default: return fail(TRACE_E_UNSUPPORTED, "unknown option");
   }
 }

@@ -12,6 +12,7 @@
 
 #include "../../include/trace_hip.h"
 #include "internal.h"
+#include "staged.h"
 
 namespace trace_hip {
 
@@ -108,6 +109,9 @@ struct trace_handle {
   int fwd_sb = 16;        // slices per forward lane (xb layout)
   int fwd_bundle = 8;     // adjacent angles per forward wavefront
   int fwd_lanes = 1;      // lanes per ray of the forward projector
+  int fwd_staged = 1;     // TRACE_OPT_FWD_STAGED
+  trace_hip::StagedPlan staged;  // slab plan of the staged forward projector
+  bool staged_ready = false;
   trace_hip::TileShape tile{16, 16, 16};
   trace_hip::RayInfo *rays = nullptr;     // [P][C]
   bool rays_ready = false;

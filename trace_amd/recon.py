@@ -190,6 +190,12 @@ class Projector:
         """Tile (default) or per-pixel form of the ordered sums (same bits)."""
         check(lib().trace_set_option(self._h, _capi.TRACE_OPT_ORDERED_TILE, 1 if on else 0))
 
+    def set_fwd_staged(self, on: bool):
+        """Slab-staged (LDS, the default) or whole-ray form of the forward
+        projector at 32-slice batches (TRACE_OPT_FWD_STAGED; same bits either
+        way)."""
+        check(lib().trace_set_option(self._h, _capi.TRACE_OPT_FWD_STAGED, 1 if on else 0))
+
     def get_option(self, option: int) -> int:
         v = ct.c_int()
         check(lib().trace_get_option(self._h, int(option), ct.byref(v)))
