@@ -10,7 +10,8 @@ oracle's fp32 result and of its fp64 twin.
          25 iterations, through the batched graph path
   stack_mlem, stack_sirt
          MLEM and SIRT at the benchmark's geometry (2048^2, 1500 angles),
-         10 iterations
+         10 iterations; these and cfg2p also as stacks of 64 (32) copies of
+         the slice, which run the 32-slice kernels of the benchmark
 
 Bar: relative L2 <= 1e-4 against the reference on the sampled pixels and on
 the row sums.  PML and APMLR sum den and num in the reference's order, so
@@ -82,6 +83,36 @@ def test_config_vs_reference(gpu, key):
         pj.reconstruct(meta["alg"], meta["iters"], ds, dx, rp)
         torch.cuda.synchronize()
         x = dx.cpu().numpy()
+    _check(key, x, meta, arr, exact=meta["alg"] in ("pml", "apmlr"))
+
+
+@pytest.mark.parametrize("key,S", [("stack_mlem", 64), ("stack_sirt", 64), ("cfg2p", 32)])
+def test_stack_form_vs_reference(gpu, key, S):
+    """The benchmark's stack form on S copies of the case's slice: 32-slice
+    batches, so the slab-staged forward projector and 16x16x32 backprojector
+    tiles run (a one-slice stack runs the one-slice kernels).  Every slice
+    must equal the first bit for bit, and the first meets the case's bar
+    against the reference (PML: bit-identical sampled pixels)."""
+    import torch
+    from trace_amd import Projector, RegParams
+    cases, arr = _fixture()
+    meta = cases[key]
+    print(f"case {key} as a stack of {S}: building inputs", flush=True)
+    th, sino, init = _inputs(key, meta)
+    P, _, C = sino.shape
+    prm = meta["params"]
+    rp = RegParams(**prm) if prm else None
+    with Projector(th, S, C, C / 2.0) as pj:
+        ds = torch.from_numpy(sino).cuda().expand(P, S, C).contiguous()
+        dx = torch.from_numpy(init).cuda().expand(S, C, C).contiguous()
+        assert pj.launch_shape()[0] == 32
+        pj.reconstruct(meta["alg"], meta["iters"], ds, dx, rp)
+        torch.cuda.synchronize()
+        bits = dx.view(torch.int32)
+        same = [bool(torch.equal(bits[k], bits[0])) for k in range(S)]
+        x = dx[:1].cpu().numpy()
+    print(f"{key}: {sum(same)} of {S} slices equal the first", flush=True)
+    assert all(same)
     _check(key, x, meta, arr, exact=meta["alg"] in ("pml", "apmlr"))
 
 
