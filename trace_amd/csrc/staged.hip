@@ -238,7 +238,8 @@ __device__ __forceinline__ void forward_staged(
   __syncthreads();
   // Slab k: stage slab k + 1 into the other buffer while the lanes add their
   // segments up to slab k (and those of earlier slabs, from HBM: they were
-  // not staged).  The pass k == nslab adds what is left, from HBM (nothing
+  // not staged).  The pass k == nslab, past the last slab (every segment in
+  // the image has a slab below nslab), adds what is left, from HBM (nothing
   // for a ray inside the grid).
   for (int k = 0; k <= nslab; ++k) {
     const bool more = k + 1 < nslab;
@@ -249,7 +250,7 @@ __device__ __forceinline__ void forward_staged(
       so[j] = more ? stage_src(threadIdx.x + j * kBlock, k + 1, nxt.x, nxt.y, N, axis) : -1;
       if (so[j] >= 0) stg[j] = *reinterpret_cast<const f4 *>(xbb + so[j]);
     }
-    const int kk = k < nslab ? k : (1 << 30);
+    const int kk = k;
     const int lo = cur.x, E = k < nslab ? cur.y : 0;
     const int base = (k & 1) * kBuf4;
     while (r.has && r.key <= kk) {
