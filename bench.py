@@ -29,6 +29,9 @@ byte model of a stored operator bounds them; what they move is measured:
               per slice batch; traffic_over_compulsory is the re-read factor;
   limiter   = the binding resource found from counters (profiles/
               counters.json) with its utilisation;
+  valu_issue_frac = that kernel's wave64 VALU instructions per SIMD clock
+              over the 0.25 a SIMD retires: the ray kernels are bound by
+              VALU issue, not by HBM;
   stored_operator_equiv_GBs = the byte model of streaming a stored A once
               per product (8 B per nonzero: utils/inters.py:11-15 counts the
               nonzeros) over the same time, reported for comparison only.
@@ -479,6 +482,9 @@ def run_stack(args):
             "compulsory_bytes_per_launch": kd["compulsory_bytes_per_launch"],
             "traffic_over_compulsory": kd["traffic_over_compulsory"],
             "limiter": limiter,
+            # the binding resource of the ray kernels: wave64 VALU instructions
+            # per SIMD clock over the 0.25 a SIMD retires (counters)
+            "valu_issue_frac": ((crec or {}).get("derived") or {}).get("valu_issue_share"),
             "stored_operator_bytes_per_launch": kd["stored_operator_bytes_per_launch"],
             "stored_operator_model": ("per slice 8*nnz + 8*R + 4*N^2 (forward) / 8*nnz + 4*R "
                                       "+ 4*N^2 (back): an int32 index and an fp32 length per "
