@@ -15,6 +15,9 @@ test can tell that it rebuilt the same input):
   cfg2a   APMLR, beta0 0.1, delta0 0.01, beta1 = delta1 = regw = 1, compat
           slice offset, 2048^2, 1500 angles, 30 iterations, a 2-slice stack
           (apmlr_main.cc:209-244; APMLR couples slices)
+  cfg2a_fixed
+          the same with the slice offset corrected (the oracle's restatement
+          of a mode the reference does not have)
   cfg3_i  slice i of the 200 uneven slices (trace_amd.batch.uneven_specs,
           seed 3): the first of width 256, 1024, 1536 and 2048, 25 iterations
   stack_mlem, stack_sirt
@@ -52,6 +55,8 @@ CASES = {
     "cfg1": ("mlem", 1024, 720, 1, 50, 1, 1e5, 20, None),
     "cfg2p": ("pml", 2048, 1500, 1, 30, 2, 1e4, 20, dict(beta=0.1)),
     "cfg2a": ("apmlr", 2048, 1500, 2, 30, 2, 1e4, 20, APMLR_PRM),
+    # APMLR with the ghost offset corrected (no reference counterpart)
+    "cfg2a_fixed": ("apmlr", 2048, 1500, 2, 30, 2, 1e4, 20, dict(APMLR_PRM, apmlr_compat=0)),
     # the benchmark's geometry and algorithms (its slice-0 phantom)
     "stack_mlem": ("mlem", 2048, 1500, 1, 10, 2, 1e4, 20, None),
     "stack_sirt": ("sirt", 2048, 1500, 1, 10, 2, 1e4, 20, None),
