@@ -21,6 +21,9 @@ GEOMS = [
     (64, 96, 37, 48.0, 32),      # detector wider than the image
     (64, 64, 30, 0.0, 32),       # the reference's default centre
     (300, 300, 181, 150.0, 64),  # many slabs and bundles, two batches
+    (8, 8, 5, 4.0, 32),          # tiny: two slabs, a bundle of 5 angles
+    (5, 7, 9, 3.5, 32),          # odd side (a partial last slab), detector wider
+    (130, 70, 17, 35.0, 32),     # bundles cut at 16 angles and 64 columns
 ]
 
 
