@@ -92,3 +92,23 @@ def test_recon_staged_bit_identical(gpu, alg):
     bad = int(np.count_nonzero(_bits(out[0]) != _bits(out[1])))
     print(f"{alg}: {bad} of {out[0].size} pixels differ")
     assert bad == 0
+
+
+def test_forced_32_slice_batch_with_fewer_slices(gpu):
+    """A 32-slice forward batch chosen for a 5-slice stack: the staged form
+    reads whole 32-slot lines and writes only the live slices."""
+    import torch
+    from trace_amd import Projector
+    N, P, S = 64, 40, 5
+    th = _angles("random", P, 11)
+    x = np.random.default_rng(5).uniform(0, 1, (S, N, N)).astype(np.float32)
+    print(f"staged forward, forced 32-slice batch, S={S}", flush=True)
+    ys = []
+    for staged in (True, False):
+        with Projector(th, S, N, N / 2.0) as pj:
+            pj.set_launch_shape(32, 16, 16, 32)
+            pj.set_fwd_staged(staged)
+            y = pj.forward(torch.from_numpy(x).cuda())
+            torch.cuda.synchronize()
+            ys.append(y.cpu().numpy())
+    assert np.array_equal(_bits(ys[0]), _bits(ys[1]))
