@@ -70,7 +70,7 @@ def test_forward_staged_vs_oracle(gpu, oracle):
     assert np.array_equal(_bits(ys), _bits(oracle.forward(th, x, N / 2)))
 
 
-@pytest.mark.parametrize("alg", ["sirt", "mlem", "pml"])
+@pytest.mark.parametrize("alg", ["sirt", "mlem", "pml", "apmlr"])
 def test_recon_staged_bit_identical(gpu, alg):
     import torch
     from trace_amd import Projector, RegParams
