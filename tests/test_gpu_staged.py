@@ -112,3 +112,16 @@ def test_forced_32_slice_batch_with_fewer_slices(gpu):
             torch.cuda.synchronize()
             ys.append(y.cpu().numpy())
     assert np.array_equal(_bits(ys[0]), _bits(ys[1]))
+
+
+def test_staged_option(gpu):
+    """TRACE_OPT_FWD_STAGED: on by default, 0 or 1 only."""
+    from trace_amd import Projector, _capi
+    from trace_amd.phantom import angles_rad
+    with Projector(angles_rad(8), 32, 16, 8.0) as pj:
+        assert pj.get_option(_capi.TRACE_OPT_FWD_STAGED) == 1
+        pj.set_fwd_staged(False)
+        assert pj.get_option(_capi.TRACE_OPT_FWD_STAGED) == 0
+        rc = _capi.lib().trace_set_option(pj.handle, _capi.TRACE_OPT_FWD_STAGED, 2)
+        assert rc == _capi.TRACE_E_INVALID
+        assert pj.get_option(_capi.TRACE_OPT_FWD_STAGED) == 0
