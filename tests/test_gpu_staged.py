@@ -125,3 +125,22 @@ def test_staged_option(gpu):
         rc = _capi.lib().trace_set_option(pj.handle, _capi.TRACE_OPT_FWD_STAGED, 2)
         assert rc == _capi.TRACE_E_INVALID
         assert pj.get_option(_capi.TRACE_OPT_FWD_STAGED) == 0
+
+
+def test_forward_staged_fast_sqrt(gpu):
+    """Hardware square roots (TRACE_OPT_EXACT_SQRT = 0): the two forms still
+    agree bit for bit."""
+    import torch
+    from trace_amd import Projector
+    N, P, S = 96, 60, 32
+    th = _angles("random", P, 13)
+    x = np.random.default_rng(9).uniform(0, 1, (S, N, N)).astype(np.float32)
+    print(f"staged forward, hardware sqrt, N={N} P={P} S={S}", flush=True)
+    ys = []
+    for staged in (True, False):
+        with Projector(th, S, N, N / 2.0, exact_sqrt=False) as pj:
+            pj.set_fwd_staged(staged)
+            y = pj.forward(torch.from_numpy(x).cuda())
+            torch.cuda.synchronize()
+            ys.append(y.cpu().numpy())
+    assert np.array_equal(_bits(ys[0]), _bits(ys[1]))
