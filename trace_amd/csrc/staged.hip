@@ -129,13 +129,14 @@ k_staged_plan(GridInfo g, const RayInfo *__restrict__ rays, int P, int nslab, in
 }
 
 // The walk of one lane, one segment ahead: the pending segment is computed
-// but not yet added (idx, in, len, len2; key its slab, -1 to take it at
-// once; u, v its slab coordinates).
+// but not yet added (idx, len, len2; u, v its slab coordinates; key its
+// slab, -1 for a pixel outside the image, taken at once and adding nothing,
+// and kStDone once the walk is over).
+constexpr int kStDone = 1 << 30;
 struct StagedWalk {
   Walker w;
   float px, py;
   int rem;
-  bool has, in;
   unsigned idx;
   float len, len2;
   int key, u, v;
@@ -144,7 +145,7 @@ struct StagedWalk {
 template <bool kExact, int AXIS>
 __device__ __forceinline__ void staged_advance(StagedWalk &r, const GridInfo &g, unsigned NN) {
   if (r.rem <= 0) {
-    r.has = false;
+    r.key = kStDone;
     return;
   }
   const int N = g.N;
@@ -156,14 +157,14 @@ __device__ __forceinline__ void staged_advance(StagedWalk &r, const GridInfo &g,
   r.py = qy;
   --r.rem;
   r.idx = (unsigned)(ix + iy * N);
-  r.in = r.idx < NN;
+  const bool in = r.idx < NN;
   int row = iy, col = ix;
   if ((unsigned)ix >= (unsigned)N) {  // a wrap onto the next row
     row = (int)(r.idx / (unsigned)N);
     col = (int)(r.idx - (unsigned)row * (unsigned)N);
   }
   slab_uv(AXIS, N, row, col, r.u, r.v);
-  r.key = r.in ? (int)((unsigned)r.u / kStH) : -1;
+  r.key = in ? (int)((unsigned)r.u / kStH) : -1;
 }
 
 // Global offset (in floats) of staged float4 f of slab k, or -1 if it holds
@@ -210,10 +211,10 @@ __device__ __forceinline__ void forward_staged(
   StagedWalk r;
   r.px = r.py = 0.f;
   r.rem = 0;
-  r.has = r.in = false;
   r.idx = 0;
   r.len = r.len2 = 0.f;
-  r.key = r.u = r.v = 0;
+  r.key = kStDone;
+  r.u = r.v = 0;
   const int64_t t = (int64_t)p * C + c;
   if (valid) {
     r.w.init(rays[t], N, ang[p].quadrant);
@@ -222,7 +223,6 @@ __device__ __forceinline__ void forward_staged(
     if (n >= 2) {
       r.w.next(r.px, r.py);
       r.rem = n - 1;
-      r.has = true;
       staged_advance<kExact, AXIS>(r, g, NN);
     }
   }
@@ -253,8 +253,8 @@ __device__ __forceinline__ void forward_staged(
     const int kk = k;
     const int lo = cur.x, E = k < nslab ? cur.y : 0;
     const int base = (k & 1) * kBuf4;
-    while (r.has && r.key <= kk) {
-      if (r.in) {
+    while (r.key <= kk) {
+      if (r.key >= 0) {
         const int dv = r.v - lo;
         const bool staged = r.key == kk && (unsigned)dv < (unsigned)E;
         const int q = (r.u - kk * kStH) * E + dv;
@@ -292,7 +292,7 @@ __device__ __forceinline__ void forward_staged(
           }
         }
       }
-      hit |= r.in;
+      hit |= r.key >= 0;
       if (MODE == kFwdOutSIRT) a2 += r.len2;
       staged_advance<kExact, AXIS>(r, g, NN);
     }
