@@ -143,7 +143,9 @@ struct Walker {
   bool fwd;
   bool whole;           // windows are unsafe for this ray: walk it whole
   bool one_axis;        // only the primary axis may bound its windows
-  int i, j, nA, nB;     // A/B elements emitted so far; grid lines of the heads
+  int i, j;             // A/B elements before the position seek() set
+  int nA, nB;           // grid lines of the heads (next() advances these only)
+  int nAend, nBend;     // the head lines once a list is exhausted
   float ax, ay, bx, by; // heads (computed even past the end, never selected)
 
   __device__ __forceinline__ void init(const RayInfo &r, int N_, int quadrant) {
@@ -153,6 +155,8 @@ struct Walker {
     one_axis = (r.alo & kRay1D) != 0;
     fwd = quadrant != 0;  // A walked forward iff quadrant == 1 (:175,183,197)
     dA = fwd ? 1 : -1;
+    nAend = fwd ? alo + alen : alo - 1;  // a_line(alen)
+    nBend = blo + blen;
   }
   __device__ __forceinline__ int a_line(int wi) const {
     return fwd ? alo + wi : alo + alen - 1 - wi;
@@ -211,14 +215,19 @@ struct Walker {
     while (grid_line(N, n) > v) --n;       // stops at blo: v >= gl(blo)
     return n - blo + 1;
   }
-  // Emits the next merged element: A wins only on a strict ax < bx
-  // (trace_utils.cc:184).  Branch-free across the wavefront.
+  // Whether the next merged element is A's head: A wins only on a strict
+  // ax < bx (trace_utils.cc:184).  The heads' lines tell whether a list is
+  // exhausted (i == alen iff nA == a_line(alen); the counted walks never
+  // take more than alen + blen elements, so j <= blen and j == blen iff
+  // nB == blo + blen): no element counters are kept per step.
+  __device__ __forceinline__ bool take_a() const {
+    return (nA != nAend) && (nB == nBend || ax < bx);
+  }
+  // Emits the next merged element.  Branch-free across the wavefront.
   __device__ __forceinline__ void next(float &qx, float &qy) {
-    const bool takeA = (i < alen) && (j >= blen || ax < bx);
+    const bool takeA = take_a();
     qx = takeA ? ax : bx;
     qy = takeA ? ay : by;
-    i += takeA ? 1 : 0;
-    j += takeA ? 0 : 1;
     nA += takeA ? dA : 0;
     nB += takeA ? 0 : 1;
     const int nn = takeA ? nA : nB;
@@ -360,7 +369,7 @@ __device__ __forceinline__ int ray_flags(const GridInfo &g, const RayInfo &r, in
   float px = 0.f;
   const int n = w.alen + w.blen;
   for (int k = 0; k < n; ++k) {
-    const bool takeA = (w.i < w.alen) && (w.j >= w.blen || w.ax < w.bx);
+    const bool takeA = w.take_a();
     float qx, qy;
     w.next(qx, qy);
     if (k > 0) {
